@@ -1,0 +1,349 @@
+"""arrayfire_amd: ArrayFire's convolution path on MI355X (gfx950).
+
+The product is ``lib/libafhip.so``: the af_* C ABI (include/af/*.h) over
+hand-written gfx950 kernels. This module is its Python host mirror, shaped
+like ArrayFire's C++ API for the same path (src/api/cpp/convolve.cpp:20-90):
+
+    convolve(signal, filter, mode, domain)      -> af::convolve (rank = min ndims)
+    convolve1/2/3(signal, filter, mode, domain)  -> af::convolve1/2/3
+    convolve_sep(col_filter, row_filter, signal, mode)
+                                                 -> af::convolve(col, row, signal)
+    convolve2NN(signal, filter, stride, padding, dilation)
+    convolve2GradientNN(grad, signal, filter, output, stride, padding,
+                        dilation, grad_type)
+
+Arguments and results are ``Array`` handles (af_array owners). Errors raise
+``AfError`` carrying the af_err code, as AF_THROW raises af::exception.
+Loading fails loudly when the library has not been built: there is no CPU
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libafhip.so")
+
+# af_dtype (include/af/defines.h)
+f32, c32, f64, c64, b8, s32, u32, u8, s64, u64, s16, u16, f16, s8 = range(14)
+bf16 = 64  # afhip_bf16 (include/af/hip.h)
+
+AF_CONV_DEFAULT, AF_CONV_EXPAND = 0, 1
+AF_CONV_AUTO, AF_CONV_SPATIAL, AF_CONV_FREQ = 0, 1, 2
+AF_CONV_GRADIENT_DEFAULT, AF_CONV_GRADIENT_FILTER, AF_CONV_GRADIENT_DATA, \
+    AF_CONV_GRADIENT_BIAS = 0, 1, 2, 3
+PRECISION_DEFAULT, PRECISION_BF16, PRECISION_F16, PRECISION_SPLIT_BF16, \
+    PRECISION_F32 = 0, 1, 2, 3, 4
+
+AF_SUCCESS = 0
+AF_ERR_ARG, AF_ERR_SIZE, AF_ERR_TYPE = 202, 203, 204
+AF_ERR_NOT_SUPPORTED, AF_ERR_INTERNAL = 301, 998
+
+_NP2AF = {np.dtype(np.float32): f32, np.dtype(np.complex64): c32,
+          np.dtype(np.float64): f64, np.dtype(np.complex128): c64,
+          np.dtype(np.bool_): b8, np.dtype(np.int32): s32,
+          np.dtype(np.uint32): u32, np.dtype(np.uint8): u8,
+          np.dtype(np.int64): s64, np.dtype(np.uint64): u64,
+          np.dtype(np.int16): s16, np.dtype(np.uint16): u16,
+          np.dtype(np.float16): f16, np.dtype(np.int8): s8}
+_AF2NP = {v: k for k, v in _NP2AF.items()}
+_AF2NP[bf16] = np.dtype(np.uint16)  # raw bfloat16 bits
+
+EXPORTED = [
+    # include/af/array.h
+    "af_create_array", "af_get_data_ptr", "af_release_array", "af_retain_array",
+    "af_eval", "af_get_elements", "af_get_type", "af_get_dims", "af_get_numdims",
+    "af_constant", "af_get_last_error",
+    # include/af/device.h
+    "af_init", "af_get_device_count", "af_set_device", "af_get_device", "af_sync",
+    "af_device_array", "af_get_device_ptr", "af_alloc_host", "af_free_host",
+    "af_device_gc", "af_device_mem_info",
+    # include/af/signal.h
+    "af_convolve1", "af_convolve2", "af_convolve3", "af_convolve2_sep",
+    "af_fft_convolve1", "af_fft_convolve2", "af_fft_convolve3",
+    # include/af/ml.h
+    "af_convolve2_nn", "af_convolve2_gradient_nn",
+    # include/af/hip.h
+    "afhip_get_stream", "af_cast", "afhip_convolve2_nn_ex",
+    "afhip_convolve2_gradient_nn_ex",
+    # include/afhip_kernels.h
+    "afhip_k_convolve", "afhip_k_convolve2_sep", "afhip_k_cast", "afhip_k_fill",
+    "afhip_k_scale", "afhip_k_conv2_nn_fwd", "afhip_k_conv2_nn_dgrad",
+    "afhip_k_conv2_nn_wgrad_workspace", "afhip_k_conv2_nn_wgrad",
+    "afhip_k_conv2_nn_f64",
+]
+
+
+class AfError(RuntimeError):
+    def __init__(self, code: int, msg: str = ""):
+        super().__init__(f"af_err {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded libafhip.so. Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with "
+                "`python -m arrayfire_amd.build` (there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i64p = ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)
+        u = ctypes.c_uint
+        L.af_get_last_error.restype = None
+        L.af_get_last_error.argtypes = [ctypes.POINTER(ctypes.c_char_p),
+                                        ctypes.POINTER(ctypes.c_longlong)]
+        L.af_free_host.argtypes = [vp]
+        L.af_create_array.argtypes = [ctypes.POINTER(vp), vp, u, i64p, ctypes.c_int]
+        L.af_constant.argtypes = [ctypes.POINTER(vp), ctypes.c_double, u, i64p, ctypes.c_int]
+        L.af_get_data_ptr.argtypes = [vp, vp]
+        L.af_release_array.argtypes = [vp]
+        L.af_retain_array.argtypes = [ctypes.POINTER(vp), vp]
+        L.af_get_type.argtypes = [ctypes.POINTER(ctypes.c_int), vp]
+        L.af_get_dims.argtypes = [i64p, i64p, i64p, i64p, vp]
+        L.af_get_numdims.argtypes = [ctypes.POINTER(u), vp]
+        L.af_get_elements.argtypes = [i64p, vp]
+        L.af_sync.argtypes = [ctypes.c_int]
+        L.afhip_get_stream.argtypes = [ctypes.POINTER(vp), ctypes.c_int]
+        L.af_get_device_ptr.argtypes = [ctypes.POINTER(vp), vp]
+        L.af_device_array.argtypes = [ctypes.POINTER(vp), vp, u, i64p, ctypes.c_int]
+        L.af_cast.argtypes = [ctypes.POINTER(vp), vp, ctypes.c_int]
+        for n in ("af_convolve1", "af_convolve2", "af_convolve3"):
+            getattr(L, n).argtypes = [ctypes.POINTER(vp), vp, vp, ctypes.c_int, ctypes.c_int]
+        for n in ("af_fft_convolve1", "af_fft_convolve2", "af_fft_convolve3"):
+            getattr(L, n).argtypes = [ctypes.POINTER(vp), vp, vp, ctypes.c_int]
+        L.af_convolve2_sep.argtypes = [ctypes.POINTER(vp), vp, vp, vp, ctypes.c_int]
+        L.af_convolve2_nn.argtypes = [ctypes.POINTER(vp), vp, vp, u, i64p, u, i64p, u, i64p]
+        L.afhip_convolve2_nn_ex.argtypes = [ctypes.POINTER(vp), vp, vp, u, i64p, u, i64p, u,
+                                            i64p, ctypes.c_int, ctypes.c_int]
+        L.af_convolve2_gradient_nn.argtypes = [ctypes.POINTER(vp), vp, vp, vp, vp, u, i64p, u,
+                                               i64p, u, i64p, ctypes.c_int]
+        L.afhip_convolve2_gradient_nn_ex.argtypes = [ctypes.POINTER(vp), vp, vp, vp, vp, u, i64p,
+                                                     u, i64p, u, i64p, ctypes.c_int, ctypes.c_int,
+                                                     ctypes.c_int]
+        ll = ctypes.c_longlong
+        L.afhip_k_convolve.argtypes = [ctypes.c_int, vp, i64p, vp, i64p, vp, i64p, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int, vp]
+        L.afhip_k_convolve2_sep.argtypes = [ctypes.c_int, vp, i64p, vp, i64p, vp, ll, vp, ll,
+                                            ctypes.c_int, vp, vp]
+        L.afhip_k_cast.argtypes = [ctypes.c_int, vp, ctypes.c_int, vp, ll, vp]
+        L.afhip_k_fill.argtypes = [ctypes.c_int, vp, ctypes.c_double, ll, vp]
+        for n in ("afhip_k_conv2_nn_fwd", "afhip_k_conv2_nn_dgrad"):
+            getattr(L, n).argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp, i64p,
+                                      i64p, i64p, i64p, i64p, i64p, vp]
+        L.afhip_k_conv2_nn_wgrad_workspace.argtypes = [i64p] * 6
+        L.afhip_k_conv2_nn_wgrad_workspace.restype = ll
+        L.afhip_k_conv2_nn_wgrad.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp,
+                                             i64p, i64p, i64p, i64p, i64p, i64p, vp, ll, vp]
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    msg = ctypes.c_char_p()
+    n = ctypes.c_longlong()
+    lib().af_get_last_error(ctypes.byref(msg), ctypes.byref(n))
+    if not msg.value:
+        return ""
+    s = msg.value.decode(errors="replace")
+    lib().af_free_host(ctypes.cast(msg, ctypes.c_void_p))
+    return s
+
+
+def check(code: int) -> None:
+    if code != AF_SUCCESS:
+        raise AfError(code, last_error())
+
+
+def dims_buf(d) -> ctypes.Array:
+    d = list(d)
+    return (ctypes.c_longlong * max(1, len(d)))(*d)
+
+
+class Array:
+    """Owner of one af_array handle (released on garbage collection)."""
+
+    __slots__ = ("_h",)
+
+    def __init__(self, data=None, dims=None, dtype=None, *, handle=None):
+        self._h = ctypes.c_void_p(handle)
+        if handle is not None:
+            return
+        a = np.asarray(data)
+        if dtype == bf16:
+            if a.dtype != np.uint16:
+                raise TypeError("bf16 arrays are created from uint16 bit patterns")
+            t = bf16
+        else:
+            if a.dtype not in _NP2AF:
+                a = a.astype(np.float32)
+            t = _NP2AF[a.dtype]
+        if dims is None:
+            dims = a.shape if a.ndim else (1,)
+        dims = tuple(int(v) for v in dims)
+        if len(dims) > 4:
+            raise ValueError("at most 4 dims")
+        flat = np.ascontiguousarray(a.reshape(-1, order="F")) if a.ndim else a.reshape(1)
+        if flat.dtype == np.bool_:
+            flat = flat.view(np.int8)
+        h = ctypes.c_void_p()
+        check(lib().af_create_array(ctypes.byref(h), flat.ctypes.data_as(ctypes.c_void_p),
+                                    len(dims), dims_buf(dims), t))
+        self._h = h
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        return self._h
+
+    def __del__(self):
+        if _lib is not None and self._h and self._h.value:
+            _lib.af_release_array(self._h)
+            self._h = ctypes.c_void_p()
+
+    @property
+    def dims(self) -> tuple:
+        d = [ctypes.c_longlong() for _ in range(4)]
+        check(lib().af_get_dims(*[ctypes.byref(x) for x in d], self._h))
+        return tuple(x.value for x in d)
+
+    @property
+    def type(self) -> int:
+        t = ctypes.c_int()
+        check(lib().af_get_type(ctypes.byref(t), self._h))
+        return t.value
+
+    def numdims(self) -> int:
+        n = ctypes.c_uint()
+        check(lib().af_get_numdims(ctypes.byref(n), self._h))
+        return n.value
+
+    def elements(self) -> int:
+        d = self.dims
+        return d[0] * d[1] * d[2] * d[3]
+
+    def device_ptr(self) -> int:
+        p = ctypes.c_void_p()
+        check(lib().af_get_device_ptr(ctypes.byref(p), self._h))
+        return p.value or 0
+
+    def to_numpy(self) -> np.ndarray:
+        """Host copy with shape (d0, d1, d2, d3), element [i0,i1,i2,i3]."""
+        d = self.dims
+        dt = _AF2NP[self.type]
+        out = np.empty(d[0] * d[1] * d[2] * d[3], dtype=np.int8 if dt == np.bool_ else dt)
+        if out.size:
+            check(lib().af_get_data_ptr(out.ctypes.data_as(ctypes.c_void_p), self._h))
+        if dt == np.bool_:
+            out = out.view(np.bool_)
+        return out.reshape(d, order="F")
+
+    def astype(self, t: int) -> "Array":
+        h = ctypes.c_void_p()
+        check(lib().af_cast(ctypes.byref(h), self._h, t))
+        return Array(handle=h.value)
+
+
+def constant(value: float, dims, dtype: int = f32) -> Array:
+    h = ctypes.c_void_p()
+    check(lib().af_constant(ctypes.byref(h), float(value), len(dims), dims_buf(dims), dtype))
+    return Array(handle=h.value)
+
+
+def sync() -> None:
+    check(lib().af_sync(-1))
+
+
+def stream() -> int:
+    """The library's HIP stream (afhip_get_stream)."""
+    s = ctypes.c_void_p()
+    check(lib().afhip_get_stream(ctypes.byref(s), 0))
+    return s.value or 0
+
+
+def _out(fn, *args) -> Array:
+    h = ctypes.c_void_p()
+    check(fn(ctypes.byref(h), *args))
+    return Array(handle=h.value)
+
+
+def _arr(a) -> Array:
+    return a if isinstance(a, Array) else Array(a)
+
+
+# ------------------------------------------------ af:: convolution functions
+# Inputs converted from numpy are bound to locals so that their handles stay
+# alive for the duration of the call.
+def convolve1(signal, filter, mode=AF_CONV_DEFAULT, domain=AF_CONV_AUTO) -> Array:
+    s, f = _arr(signal), _arr(filter)
+    return _out(lib().af_convolve1, s.handle, f.handle, mode, domain)
+
+
+def convolve2(signal, filter, mode=AF_CONV_DEFAULT, domain=AF_CONV_AUTO) -> Array:
+    s, f = _arr(signal), _arr(filter)
+    return _out(lib().af_convolve2, s.handle, f.handle, mode, domain)
+
+
+def convolve3(signal, filter, mode=AF_CONV_DEFAULT, domain=AF_CONV_AUTO) -> Array:
+    s, f = _arr(signal), _arr(filter)
+    return _out(lib().af_convolve3, s.handle, f.handle, mode, domain)
+
+
+def convolve(signal, filter, mode=AF_CONV_DEFAULT, domain=AF_CONV_AUTO) -> Array:
+    """af::convolve: rank = min(numdims) (src/api/cpp/convolve.cpp:20-32)."""
+    s, f = _arr(signal), _arr(filter)
+    r = min(s.numdims(), f.numdims())
+    if r == 1:
+        return convolve1(s, f, mode, domain)
+    if r == 2:
+        return convolve2(s, f, mode, domain)
+    return convolve3(s, f, mode, domain)
+
+
+def convolve_sep(col_filter, row_filter, signal, mode=AF_CONV_DEFAULT) -> Array:
+    c, r, s = _arr(col_filter), _arr(row_filter), _arr(signal)
+    return _out(lib().af_convolve2_sep, c.handle, r.handle, s.handle, mode)
+
+
+def _two(v) -> ctypes.Array:
+    v = tuple(v)
+    return dims_buf(v)
+
+
+def convolve2NN(signal, filter, stride=(1, 1), padding=(0, 0), dilation=(1, 1),
+                precision: int | None = None, out_type: int | None = None) -> Array:
+    """af::convolve2NN; like the C++ wrapper it passes 2 dims of each
+    parameter (src/api/cpp/convolve.cpp:55-63). ``precision``/``out_type``
+    select afhip_convolve2_nn_ex."""
+    s, f = _arr(signal), _arr(filter)
+    st, pd, dl = _two(stride), _two(padding), _two(dilation)
+    if precision is None and out_type is None:
+        return _out(lib().af_convolve2_nn, s.handle, f.handle, len(stride), st, len(padding), pd,
+                    len(dilation), dl)
+    return _out(lib().afhip_convolve2_nn_ex, s.handle, f.handle, len(stride), st, len(padding),
+                pd, len(dilation), dl, precision or 0, -1 if out_type is None else out_type)
+
+
+def convolve2GradientNN(incoming_gradient, original_signal, original_filter, convolved_output,
+                        stride=(1, 1), padding=(0, 0), dilation=(1, 1),
+                        grad_type=AF_CONV_GRADIENT_DEFAULT, precision: int | None = None,
+                        out_type: int | None = None) -> Array:
+    g, s, f, o = (_arr(incoming_gradient), _arr(original_signal), _arr(original_filter),
+                  _arr(convolved_output))
+    st, pd, dl = _two(stride), _two(padding), _two(dilation)
+    if precision is None and out_type is None:
+        return _out(lib().af_convolve2_gradient_nn, g.handle, s.handle, f.handle, o.handle,
+                    len(stride), st, len(padding), pd, len(dilation), dl, grad_type)
+    return _out(lib().afhip_convolve2_gradient_nn_ex, g.handle, s.handle, f.handle, o.handle,
+                len(stride), st, len(padding), pd, len(dilation), dl, grad_type,
+                precision or 0, -1 if out_type is None else out_type)
+
+
+def filter(image, kernel) -> Array:  # noqa: A001 - af::filter
+    return convolve(image, kernel, AF_CONV_DEFAULT, AF_CONV_AUTO)

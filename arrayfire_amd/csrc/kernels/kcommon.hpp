@@ -1,0 +1,78 @@
+// Shared definitions for the gfx950 kernels: element types, conversions,
+// launch helpers. Kernel sources include this file only.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace afhip {
+
+typedef long long dim_t;
+
+// Return codes of the kernel-level C ABI (values of af_err).
+enum KStatus : int {
+    K_OK = 0,
+    K_ERR_RUNTIME = 103,
+    K_ERR_ARG = 202,
+    K_ERR_SIZE = 203,
+    K_ERR_TYPE = 204,
+    K_ERR_NOT_SUPPORTED = 301,
+};
+
+// af_dtype values (include/af/defines.h) plus the bfloat16 storage type.
+enum DType : int {
+    DT_F32 = 0, DT_C32 = 1, DT_F64 = 2, DT_C64 = 3, DT_B8 = 4, DT_S32 = 5,
+    DT_U32 = 6, DT_U8 = 7, DT_S64 = 8, DT_U64 = 9, DT_S16 = 10, DT_U16 = 11,
+    DT_F16 = 12, DT_S8 = 13, DT_BF16 = 64,
+};
+
+// Complex element types with the arithmetic the spatial kernels need.
+template <typename R>
+struct cplx {
+    R re, im;
+    __host__ __device__ cplx() : re(0), im(0) {}
+    __host__ __device__ cplx(R r) : re(r), im(0) {}
+    __host__ __device__ cplx(R r, R i) : re(r), im(i) {}
+    __host__ __device__ cplx &operator+=(const cplx &o) {
+        re += o.re;
+        im += o.im;
+        return *this;
+    }
+};
+template <typename R>
+__host__ __device__ inline cplx<R> operator*(const cplx<R> &a, const cplx<R> &b) {
+    return cplx<R>(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+typedef cplx<float> cfloat;
+typedef cplx<double> cdouble;
+
+// bfloat16 / binary16 conversions (round to nearest even; NaN stays NaN).
+__host__ __device__ inline uint16_t f32_to_bf16_bits(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    uint32_t lsb = (u >> 16) & 1u;
+    u += 0x7fffu + lsb;
+    return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf16_bits_to_f32(uint16_t h) {
+    return __builtin_bit_cast(float, (uint32_t)h << 16);
+}
+
+inline int launch_status() {
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? K_OK : K_ERR_RUNTIME;
+}
+
+// Number of workgroups that fill the chip several times over; grid-stride
+// kernels cap their grid here.
+constexpr int kMaxGridBlocks = 256 * 64;
+
+// Remaps a linear workgroup id so that each of the 8 XCDs (which receive
+// workgroups round-robin) walks a contiguous range of tiles; neighbouring
+// tiles then share halo lines in that XCD's L2.
+__device__ inline long long xcd_remap(long long bid, long long nblk) {
+    const long long per = nblk / 8, rem = nblk % 8;
+    const long long xcd = bid % 8, k = bid / 8;
+    return xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
+}
+
+}  // namespace afhip

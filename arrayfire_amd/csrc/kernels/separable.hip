@@ -1,0 +1,222 @@
+// Separable 2-D convolution (af_convolve2_sep).
+//
+// Reference semantics (src/backend/cpu/kernel/convolve.hpp:213-289): a column
+// pass along dim0 into a T-typed temp, then a row pass along dim1; the tap is
+// converted to T before the product; batch over dims 2 and 3.
+//
+//  * sep_pass<InT,AccT,DIM>: one pass, one lane per output (the two-launch
+//    form, used for non-f32 types and when the caller asks for the temp).
+//  * sep_fused_f32: both passes in one kernel. A workgroup owns a
+//    64 (dim0) x 64 (dim1) output tile. It stages the (64+cf-1) x (64+rf-1)
+//    input halo in LDS, runs the column pass into a second LDS tile of
+//    64 x (64+rf-1) (rounded to float, the reference's temp precision), then
+//    the row pass to HBM: one read and one write of every element.
+#include "kcommon.hpp"
+#include "../../../include/afhip_kernels.h"
+
+namespace afhip {
+
+struct SepParams {
+    dim_t sd[4], td[4], od[4];
+    dim_t flen;
+    int expand;
+};
+
+template <typename InT, typename AccT, int DIM>
+__global__ __launch_bounds__(256) void sep_pass(InT *__restrict__ out,
+                                                const InT *__restrict__ in,
+                                                const AccT *__restrict__ filt,
+                                                SepParams p) {
+    // Pass 0 reads the signal (dims sd) and writes the temp (dims td);
+    // pass 1 reads the temp and writes the output (dims od).
+    const dim_t *id = DIM == 0 ? p.sd : p.td;
+    const dim_t *od = DIM == 0 ? p.td : p.od;
+    const dim_t n = od[0] * od[1] * od[2] * od[3];
+    const dim_t half = p.expand ? 0 : (p.flen >> 1);
+    for (dim_t lin = blockIdx.x * (dim_t)blockDim.x + threadIdx.x; lin < n;
+         lin += (dim_t)gridDim.x * blockDim.x) {
+        const dim_t i = lin % od[0];
+        dim_t r = lin / od[0];
+        const dim_t j = r % od[1];
+        const dim_t b = r / od[1];  // b2 + b3 * d2, same for all three arrays
+        const InT *ip = in + b * id[0] * id[1];
+        const dim_t ci = i + (DIM == 0 ? half : 0);
+        const dim_t cj = j + (DIM == 1 ? half : 0);
+        AccT acc = AccT(0);
+        for (dim_t f = 0; f < p.flen; ++f) {
+            const InT fv = (InT)filt[f];
+            InT s = InT(0);
+            if (DIM == 0) {
+                const dim_t x = ci - f;
+                if (x >= 0 && x < id[0] && cj < id[1]) s = ip[cj * id[0] + x];
+            } else {
+                const dim_t y = cj - f;
+                if (ci < id[0] && y >= 0 && y < id[1]) s = ip[y * id[0] + ci];
+            }
+            acc += AccT(s * fv);
+        }
+        out[lin] = (InT)acc;
+    }
+}
+
+constexpr int SF_B = 64;        // output tile edge
+constexpr int SF_MAXF = AFHIP_SEP_FUSED_MAX_TAPS;  // longest fused filter
+
+struct SepFusedParams {
+    dim_t S0, S1, O0, O1;
+    dim_t nbx, nby, nblk;
+    int cf, rf, sx, sy;
+};
+
+// Tap counts are runtime values (up to SF_MAXF); filters are staged in LDS.
+__global__ __launch_bounds__(256) void sep_fused_f32(float *__restrict__ out,
+                                                     const float *__restrict__ in,
+                                                     const float *__restrict__ cfilt,
+                                                     const float *__restrict__ rfilt,
+                                                     SepFusedParams p) {
+    constexpr int B = SF_B;
+    constexpr int LWMAX = B + SF_MAXF - 1;  // 96
+    __shared__ float xin[LWMAX * LWMAX];    // (B+cf-1) x (B+rf-1), pitch LW
+    __shared__ float tmid[B * LWMAX];       // B x (B+rf-1), pitch B
+    __shared__ float wc[SF_MAXF], wr[SF_MAXF];
+
+    const dim_t blk = xcd_remap(blockIdx.x, p.nblk);
+    const dim_t bx = blk % p.nbx;
+    const dim_t t = blk / p.nbx;
+    const dim_t by = t % p.nby;
+    const dim_t b = t / p.nby;
+    in += b * p.S0 * p.S1;
+    out += b * p.O0 * p.O1;
+
+    const int cf = p.cf, rf = p.rf;
+    const int LW = B + cf - 1, LH = B + rf - 1;
+    if (threadIdx.x < cf) wc[threadIdx.x] = cfilt[cf - 1 - threadIdx.x];
+    if (threadIdx.x >= 64 && threadIdx.x - 64 < rf) wr[threadIdx.x - 64] = rfilt[rf - 1 - (threadIdx.x - 64)];
+
+    const dim_t ox0 = bx * B, oy0 = by * B;
+    const dim_t ix0 = ox0 + p.sx - (cf - 1), iy0 = oy0 + p.sy - (rf - 1);
+    for (int idx = threadIdx.x; idx < LW * LH; idx += 256) {
+        const int r = idx / LW, c = idx - r * LW;
+        const dim_t gx = ix0 + c, gy = iy0 + r;
+        float v = 0.f;
+        if (gx >= 0 && gx < p.S0 && gy >= 0 && gy < p.S1) v = in[gy * p.S0 + gx];
+        xin[r * LW + c] = v;
+    }
+    __syncthreads();
+
+    // Column pass: tmid(x, r) = sum_k xin(x + k, r) * wc[k] (flipped taps).
+    // Lane owns column x = tid % 64 and rows r = tid/64 + 4q.
+    {
+        const int x = threadIdx.x & 63;
+        for (int r = threadIdx.x >> 6; r < LH; r += 4) {
+            float acc = 0.f;
+            const float *row = xin + r * LW + x;
+            for (int k = 0; k < cf; ++k) acc = __builtin_fmaf(row[k], wc[k], acc);
+            tmid[r * B + x] = acc;
+        }
+    }
+    __syncthreads();
+
+    // Row pass: out(x, y) = sum_k tmid(x, y + k) * wr[k]. Lane owns column x
+    // and 16 consecutive rows, sliding down the column.
+    {
+        const int x = threadIdx.x & 63;
+        const int y0 = (threadIdx.x >> 6) * 16;
+        float acc[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc[q] = 0.f;
+        for (int k = 0; k < rf; ++k) {
+            const float w = wr[k];
+#pragma unroll
+            for (int q = 0; q < 16; ++q)
+                acc[q] = __builtin_fmaf(tmid[(y0 + q + k) * B + x], w, acc[q]);
+        }
+        const dim_t gx = ox0 + x;
+        if (gx < p.O0) {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const dim_t gy = oy0 + y0 + q;
+                if (gy < p.O1) out[gy * p.O0 + gx] = acc[q];
+            }
+        }
+    }
+}
+
+static unsigned grid_for(dim_t n) {
+    dim_t b = (n + 255) / 256;
+    if (b > kMaxGridBlocks) b = kMaxGridBlocks;
+    if (b < 1) b = 1;
+    return (unsigned)b;
+}
+
+template <typename InT, typename AccT>
+static int launch_two_pass(void *out, const void *in, const void *cf, dim_t cflen,
+                           const void *rf, dim_t rflen, SepParams p, void *tmp,
+                           hipStream_t s) {
+    const dim_t nt = p.td[0] * p.td[1] * p.td[2] * p.td[3];
+    const dim_t no = p.od[0] * p.od[1] * p.od[2] * p.od[3];
+    if (no == 0) return K_OK;
+    if (!tmp) return K_ERR_ARG;
+    p.flen = cflen;
+    sep_pass<InT, AccT, 0><<<grid_for(nt), 256, 0, s>>>((InT *)tmp, (const InT *)in,
+                                                          (const AccT *)cf, p);
+    p.flen = rflen;
+    sep_pass<InT, AccT, 1><<<grid_for(no), 256, 0, s>>>((InT *)out, (const InT *)tmp,
+                                                          (const AccT *)rf, p);
+    return launch_status();
+}
+
+}  // namespace afhip
+
+using namespace afhip;
+
+extern "C" int afhip_k_convolve2_sep(int dtype, void *out, const long long *odims,
+                                     const void *in, const long long *sdims,
+                                     const void *cfilt, long long cflen,
+                                     const void *rfilt, long long rflen, int expand,
+                                     void *tmp, void *stream) {
+    if (cflen < 1 || rflen < 1) return K_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    SepParams p;
+    for (int d = 0; d < 4; ++d) {
+        p.sd[d] = sdims[d];
+        p.od[d] = odims[d];
+        p.td[d] = sdims[d];
+    }
+    if (expand) p.td[0] += cflen - 1;
+    p.expand = expand;
+    p.flen = 0;
+    if (dtype == DT_F32 && tmp == nullptr && cflen <= SF_MAXF && rflen <= SF_MAXF) {
+        SepFusedParams f;
+        f.S0 = sdims[0];
+        f.S1 = sdims[1];
+        f.O0 = odims[0];
+        f.O1 = odims[1];
+        f.nbx = (f.O0 + SF_B - 1) / SF_B;
+        f.nby = (f.O1 + SF_B - 1) / SF_B;
+        f.nblk = f.nbx * f.nby * odims[2] * odims[3];
+        f.cf = (int)cflen;
+        f.rf = (int)rflen;
+        f.sx = expand ? 0 : (int)(cflen >> 1);
+        f.sy = expand ? 0 : (int)(rflen >> 1);
+        if (f.nblk == 0) return K_OK;
+        if (f.nblk > 0x7fffffffLL) return K_ERR_SIZE;
+        sep_fused_f32<<<dim3((unsigned)f.nblk), 256, 0, s>>>(
+            (float *)out, (const float *)in, (const float *)cfilt, (const float *)rfilt, f);
+        return launch_status();
+    }
+    switch (dtype) {
+        case DT_F32: return launch_two_pass<float, float>(out, in, cfilt, cflen, rfilt, rflen, p, tmp, s);
+        case DT_F64: return launch_two_pass<double, double>(out, in, cfilt, cflen, rfilt, rflen, p, tmp, s);
+        case DT_S32: return launch_two_pass<int32_t, float>(out, in, cfilt, cflen, rfilt, rflen, p, tmp, s);
+        case DT_U32: return launch_two_pass<uint32_t, float>(out, in, cfilt, cflen, rfilt, rflen, p, tmp, s);
+        case DT_S16: return launch_two_pass<int16_t, float>(out, in, cfilt, cflen, rfilt, rflen, p, tmp, s);
+        case DT_U16: return launch_two_pass<uint16_t, float>(out, in, cfilt, cflen, rfilt, rflen, p, tmp, s);
+        case DT_S64: return launch_two_pass<long long, float>(out, in, cfilt, cflen, rfilt, rflen, p, tmp, s);
+        case DT_U64: return launch_two_pass<unsigned long long, float>(out, in, cfilt, cflen, rfilt, rflen, p, tmp, s);
+        case DT_U8: return launch_two_pass<uint8_t, float>(out, in, cfilt, cflen, rfilt, rflen, p, tmp, s);
+        case DT_S8: return launch_two_pass<int8_t, float>(out, in, cfilt, cflen, rfilt, rflen, p, tmp, s);
+        case DT_B8: return launch_two_pass<int8_t, float>(out, in, cfilt, cflen, rfilt, rflen, p, tmp, s);
+        default: return K_ERR_TYPE;
+    }
+}

@@ -1,0 +1,72 @@
+/*
+ * MI355X-specific extensions of the ArrayFire C API.
+ *
+ * afhip_get_stream mirrors afcu_get_stream (include/af/cuda.h:50). The
+ * remaining entry points have no counterpart in the reference: ArrayFire has
+ * no bfloat16 dtype (include/af/defines.h:210-233) and no reduced-precision
+ * or ragged convolution calls. They sit beside the af_* calls and share their
+ * handles, error codes and af_get_last_error.
+ */
+#pragma once
+#include <af/defines.h>
+
+/* Storage type for bfloat16 arrays (2 bytes, the upper half of an IEEE
+ * binary32). Accepted by af_create_array, af_get_data_ptr, af_cast and the
+ * convolve2_nn entry points. The value lies outside af_dtype's range so it
+ * never aliases a reference dtype. */
+#define afhip_bf16 ((af_dtype)64)
+
+/* Arithmetic used by the channel-reduction GEMM of convolve2_nn and its two
+ * adjoints. */
+typedef enum {
+    /* The dtype of the inputs decides: f32 -> FP32, f16 -> FP16 MFMA with
+     * FP32 accumulation, afhip_bf16 -> BF16 MFMA with FP32 accumulation,
+     * f64 -> FP64. This is what af_convolve2_nn does. */
+    AFHIP_PRECISION_DEFAULT = 0,
+    /* Operands rounded to bfloat16 (round to nearest even) as they are
+     * loaded, products on BF16 MFMA, FP32 accumulation. */
+    AFHIP_PRECISION_BF16 = 1,
+    /* Same with IEEE binary16 operands on FP16 MFMA. */
+    AFHIP_PRECISION_F16 = 2,
+    /* Each FP32 operand split as hi = bf16(x), lo = bf16(x - hi); three BF16
+     * MFMA products hi*hi' + hi*lo' + lo*hi' accumulated in FP32. */
+    AFHIP_PRECISION_SPLIT_BF16 = 3,
+    /* FP32 operands, FP32 products and accumulation. */
+    AFHIP_PRECISION_F32 = 4
+} afhip_precision;
+
+/* include/af/cuda.h:50 (afcu_get_stream): the HIP stream (hipStream_t,
+ * returned as void*) on which the library queues work for device id. */
+AFAPI af_err afhip_get_stream(void **stream, int id);
+
+/* include/af/arith.h (af_cast): element type conversion. Supported among
+ * f32, f64, f16, afhip_bf16 and the integer types; floating to bfloat16 and
+ * binary16 rounds to nearest even. */
+AFAPI af_err af_cast(af_array *out, const af_array in, const af_dtype type);
+
+/* af_convolve2_nn with an explicit arithmetic. signal and filter must have
+ * the same dtype (f32, f16 or afhip_bf16; f64 only with
+ * AFHIP_PRECISION_DEFAULT). out_type selects the output dtype (f32, or the
+ * input dtype when it is f16/afhip_bf16); pass (af_dtype)-1 for the default,
+ * which is the input dtype for f32/f16/f64 and f32 for afhip_bf16 inputs. */
+AFAPI af_err afhip_convolve2_nn_ex(af_array *out, const af_array signal,
+                                   const af_array filter,
+                                   const unsigned stride_dims,
+                                   const dim_t *strides,
+                                   const unsigned padding_dims,
+                                   const dim_t *paddings,
+                                   const unsigned dilation_dims,
+                                   const dim_t *dilations,
+                                   afhip_precision precision, af_dtype out_type);
+
+/* af_convolve2_gradient_nn with an explicit arithmetic (see above). The
+ * gradient is returned in out_type ((af_dtype)-1: as for
+ * afhip_convolve2_nn_ex). */
+AFAPI af_err afhip_convolve2_gradient_nn_ex(
+    af_array *out, const af_array incoming_gradient,
+    const af_array original_signal, const af_array original_filter,
+    const af_array convolved_output, const unsigned stride_dims,
+    const dim_t *strides, const unsigned padding_dims, const dim_t *paddings,
+    const unsigned dilation_dims, const dim_t *dilations,
+    af_conv_gradient_type grad_type, afhip_precision precision,
+    af_dtype out_type);

@@ -1,0 +1,125 @@
+/*
+ * Kernel-level C ABI of libafhip.so: raw device pointers, int64 dims in
+ * ArrayFire's column-major order (dim0 fastest, contiguous strides), an
+ * explicit HIP stream (hipStream_t passed as void*), and an af_err return
+ * code (0 on success; 202 bad argument, 203 bad size, 204 unsupported dtype,
+ * 103 HIP launch failure). Nothing here allocates except where a workspace
+ * is documented; no exception crosses it.
+ *
+ * These are the entry points the af_* dispatch (the headers under include/af) calls after
+ * it has validated its arguments and computed output dims. They are also
+ * what a host that manages its own device memory (the Python test wrapper,
+ * a torch.autograd.Function) binds directly. Each replaces one backend
+ * template of the reference, cited below.
+ */
+#pragma once
+
+#ifdef __cplusplus
+#define AFHIP_KAPI extern "C" __attribute__((visibility("default")))
+#else
+#define AFHIP_KAPI __attribute__((visibility("default")))
+#endif
+
+/* detail::convolve<T, accT> (src/backend/cpu/convolve.hpp:16-18;
+ * src/backend/cpu/convolve.cpp:38-66 and kernel/convolve.hpp:132-211).
+ * dtype is the signal's af_dtype; filt holds accT elements (float for f32 and
+ * the integer types, double for f64, complex float/double for c32/c64).
+ * kind is the AF_BATCH_KIND value (NONE=1, LHS=2, RHS=3, SAME=4); odims must
+ * be the output dims the reference computes for (sdims, fdims, kind, rank,
+ * expand). */
+AFHIP_KAPI int afhip_k_convolve(int dtype, void *out, const long long *odims,
+                                const void *in, const long long *sdims,
+                                const void *filt, const long long *fdims,
+                                int kind, int rank, int expand, void *stream);
+
+/* Longest column or row filter of the fused single-kernel FP32 separable
+ * path; longer filters (and every other dtype) need the tmp buffer. */
+#define AFHIP_SEP_FUSED_MAX_TAPS 33
+
+/* detail::convolve2<T, accT> separable (src/backend/cpu/convolve.hpp:20-22;
+ * src/backend/cpu/convolve.cpp:68-99, kernel/convolve.hpp:213-289).
+ * col/row filters hold cflen/rflen accT elements; odims = sdims, plus
+ * cflen-1 on dim0 and rflen-1 on dim1 when expand. tmp, when not NULL, is a
+ * device buffer of sdims[0]+(expand?cflen-1:0) x sdims[1] x sdims[2] x
+ * sdims[3] elements that receives the column pass (the reference's T-typed
+ * temp); when NULL the two passes are fused in one kernel and the
+ * intermediate is rounded to T in registers without being stored. */
+AFHIP_KAPI int afhip_k_convolve2_sep(int dtype, void *out, const long long *odims,
+                                     const void *in, const long long *sdims,
+                                     const void *cfilt, long long cflen,
+                                     const void *rfilt, long long rflen,
+                                     int expand, void *tmp, void *stream);
+
+/* Element type conversion between af_dtype values (and 64 = bfloat16) over n
+ * contiguous elements. */
+AFHIP_KAPI int afhip_k_cast(int out_type, void *out, int in_type,
+                            const void *in, long long n, void *stream);
+
+/* Fills n elements of dtype with value. */
+AFHIP_KAPI int afhip_k_fill(int dtype, void *out, double value, long long n,
+                            void *stream);
+
+/* y = T(accT(x) * scale) elementwise: the scalar/scalar case of the separable
+ * convolution (src/api/c/convolve.cpp:57-68). */
+AFHIP_KAPI int afhip_k_scale(int dtype, void *out, const void *in,
+                             double scale, long long n, void *stream);
+
+/* Precision codes below are afhip_precision values (include/af/hip.h):
+ * 0 default (by in_type), 1 BF16 MFMA, 2 FP16 MFMA, 3 split-BF16, 4 FP32.
+ * in_type is f32 (0), f16 (12) or bfloat16 (64); out_type f32, or the 16-bit
+ * input type. Dims: x [W,H,C,N], w [fw,fh,C,K], y [OW,OH,K,N]; stride, pad and
+ * dil hold two entries (dim0 = W, dim1 = H). */
+
+/* detail::convolve2<T> (src/backend/cpu/convolve.hpp:24-26;
+ * src/backend/cpu/convolve.cpp:123-167): y = conv2_nn(x, w). */
+AFHIP_KAPI int afhip_k_conv2_nn_fwd(int in_type, int out_type, int precision,
+                                    void *y, const void *x, const void *w,
+                                    const long long *xdims,
+                                    const long long *wdims,
+                                    const long long *ydims,
+                                    const long long *stride,
+                                    const long long *pad, const long long *dil,
+                                    void *stream);
+
+/* detail::conv2DataGradient<T> (src/backend/cpu/convolve.hpp:28-33;
+ * src/backend/cpu/convolve.cpp:179-211): dx [W,H,C,N] from dy [OW,OH,K,N]. */
+AFHIP_KAPI int afhip_k_conv2_nn_dgrad(int in_type, int out_type, int precision,
+                                      void *dx, const void *dy, const void *w,
+                                      const long long *xdims,
+                                      const long long *wdims,
+                                      const long long *ydims,
+                                      const long long *stride,
+                                      const long long *pad,
+                                      const long long *dil, void *stream);
+
+/* Workspace bytes afhip_k_conv2_nn_wgrad uses for its split reduction over
+ * N*OH*OW (0 when it runs unsplit). */
+AFHIP_KAPI long long afhip_k_conv2_nn_wgrad_workspace(
+    const long long *xdims, const long long *wdims, const long long *ydims,
+    const long long *stride, const long long *pad, const long long *dil);
+
+/* detail::conv2FilterGradient<T> (src/backend/cpu/convolve.hpp:35-40;
+ * src/backend/cpu/convolve.cpp:213-243): dw [fw,fh,C,K] from x and dy.
+ * workspace may be NULL (or smaller than the query): the reduction then runs
+ * unsplit, which is correct but slower. */
+AFHIP_KAPI int afhip_k_conv2_nn_wgrad(int in_type, int out_type, int precision,
+                                      void *dw, const void *x, const void *dy,
+                                      const long long *xdims,
+                                      const long long *wdims,
+                                      const long long *ydims,
+                                      const long long *stride,
+                                      const long long *pad,
+                                      const long long *dil, void *workspace,
+                                      long long workspace_bytes, void *stream);
+
+/* FP64 forward (pass 0), data gradient (pass 1) and filter gradient (pass 2)
+ * of convolve2_nn, evaluated directly from the closed forms; the f64 cases of
+ * the three templates above. Operands: pass 0 (x, w) -> y; pass 1 (dy, w) ->
+ * dx; pass 2 (x, dy) -> dw. */
+AFHIP_KAPI int afhip_k_conv2_nn_f64(int pass, void *out, const void *a0,
+                                    const void *a1, const long long *xdims,
+                                    const long long *wdims,
+                                    const long long *ydims,
+                                    const long long *stride,
+                                    const long long *pad, const long long *dil,
+                                    void *stream);

@@ -1,0 +1,461 @@
+"""CPU oracle for ArrayFire's convolution path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline
+leg may import this package, and only as the checker (or as the timed CPU
+baseline).  The product library (``arrayfire_amd``) never imports it.
+
+Two families of functions live here:
+
+* **Restatements of the reference CPU backend** (same loops, index rules,
+  accumulation types and rounding points).  The spatial/separable/im2col/col2im
+  loops are in ``conv_oracle.c`` (built by ``oracle/Makefile``); the host-side
+  composition (output dims, batch kind, im2col + SGEMM + col2im) is here:
+
+  - ``ndims``                ... src/backend/common/dim4.cpp:48-58
+  - ``identify_batch_kind``  ... src/api/c/convolve.cpp:77-97
+  - ``is_freq_domain``       ... src/api/c/convolve.cpp:99-130
+  - ``conv_out_dims``        ... src/backend/cpu/convolve.cpp:44-58
+  - ``convolve``             ... src/backend/cpu/convolve.cpp:38-66 + kernel/convolve.hpp:132-211
+  - ``convolve2_sep``        ... src/backend/cpu/convolve.cpp:68-99 + kernel/convolve.hpp:213-289
+  - ``conv2_nn``             ... src/backend/cpu/convolve.cpp:123-167
+  - ``conv2_data_grad``      ... src/backend/cpu/convolve.cpp:179-211
+  - ``conv2_filter_grad``    ... src/backend/cpu/convolve.cpp:213-243
+  - ``gaussian_kernel``      ... src/api/c/gaussian_kernel.cpp:35-88
+
+  The SGEMM the reference delegates to CBLAS (src/backend/cpu/blas.cpp:223-331,
+  a third-party library with no pinned version) is numpy's ``matmul`` on
+  float32 operands, which calls the OpenBLAS sgemm bundled with numpy.
+
+* **FP64 brute-force definitions** (``*_direct_f64``), written from the
+  closed forms of the operations (true convolution with zero borders, and its
+  two adjoints), independent of the im2col composition above.
+
+Arrays are numpy arrays of shape ``(d0, d1, d2, d3)`` whose element
+``a[i0, i1, i2, i3]`` is ArrayFire's ``a(i0, i1, i2, i3)``; flat buffers are
+taken in Fortran order, which is ArrayFire's column-major memory order.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+AF_BATCH_NONE, AF_BATCH_LHS, AF_BATCH_RHS, AF_BATCH_SAME, AF_BATCH_DIFF, \
+    AF_BATCH_UNSUPPORTED = 1, 2, 3, 4, 5, 6
+AF_CONV_AUTO, AF_CONV_SPATIAL, AF_CONV_FREQ = 0, 1, 2
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build() -> None:
+    """Compile the C restatement (liboracle_conv.so) in place."""
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle_conv.so")
+        if not os.path.exists(path):
+            build()
+        lib = ctypes.CDLL(path)
+        p = ctypes.c_void_p
+        i64 = ctypes.c_int64
+        for suf in ("f32", "f64", "s32", "u8"):
+            f = getattr(lib, f"orc_convolve_nd_{suf}")
+            f.argtypes = [p, p, p, p, p, p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+            f.restype = None
+            g = getattr(lib, f"orc_convolve2_sep_{suf}")
+            g.argtypes = [p, p, p, p, p, i64, p, i64, ctypes.c_int]
+            g.restype = ctypes.c_int
+        for suf in ("f32", "f64"):
+            u = getattr(lib, f"orc_unwrap_{suf}")
+            u.argtypes = [p, p, p] + [i64] * 8
+            u.restype = None
+            w = getattr(lib, f"orc_wrap_dilated_{suf}")
+            w.argtypes = [p, p, p, p] + [i64] * 8
+            w.restype = None
+        _LIB = lib
+    return _LIB
+
+
+# ----------------------------------------------------------------- helpers
+_SUFFIX = {np.dtype(np.float32): ("f32", np.float32),
+           np.dtype(np.float64): ("f64", np.float64),
+           np.dtype(np.int32): ("s32", np.float32),
+           np.dtype(np.uint8): ("u8", np.float32)}
+
+
+def as4(a) -> np.ndarray:
+    """View ``a`` as a 4-D ArrayFire-shaped array (trailing dims of 1)."""
+    a = np.asarray(a)
+    return a.reshape(tuple(a.shape) + (1,) * (4 - a.ndim), order="F")
+
+
+def dims4(a) -> tuple:
+    return tuple(int(v) for v in as4(a).shape)
+
+
+def _flat(a: np.ndarray, dtype) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=dtype).ravel(order="F"))
+
+
+def _dimbuf(d) -> np.ndarray:
+    return np.asarray(d, dtype=np.int64)
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def rel_l2(a, b) -> float:
+    """||a - b||_2 / ||b||_2 computed in float64."""
+    a = np.asarray(a, dtype=np.float64).ravel()
+    b = np.asarray(b, dtype=np.float64).ravel()
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
+
+
+# ----------------------------------------------------------- host logic
+def ndims(d) -> int:
+    """dim4::ndims, src/backend/common/dim4.cpp:48-58."""
+    d = tuple(d) + (1,) * (4 - len(d))
+    n = d[0] * d[1] * d[2] * d[3]
+    if n == 0:
+        return 0
+    if n == 1:
+        return 1
+    for k in (3, 2, 1):
+        if d[k] != 1:
+            return k + 1
+    return 1
+
+
+def identify_batch_kind(rank: int, sd, fd) -> int:
+    """src/api/c/convolve.cpp:77-97."""
+    sn, fn = ndims(sd), ndims(fd)
+    if sn == rank and fn == rank:
+        return AF_BATCH_NONE
+    if sn == rank and rank < fn <= 4:
+        return AF_BATCH_RHS
+    if rank < sn <= 4 and fn == rank:
+        return AF_BATCH_LHS
+    if rank < sn <= 4 and rank < fn <= 4:
+        match, inter = True, True
+        for i in range(rank, 4):
+            match &= sd[i] == fd[i]
+            inter &= sd[i] == 1 or fd[i] == 1 or sd[i] == fd[i]
+        if match:
+            return AF_BATCH_SAME
+        return AF_BATCH_DIFF if inter else AF_BATCH_UNSUPPORTED
+    return AF_BATCH_UNSUPPORTED
+
+
+def is_freq_domain(rank: int, sd, fd, domain: int) -> bool:
+    """src/api/c/convolve.cpp:99-130."""
+    if domain == AF_CONV_FREQ:
+        return True
+    if domain != AF_CONV_AUTO:
+        return False
+    if identify_batch_kind(rank, sd, fd) == AF_BATCH_DIFF:
+        return True
+    kb = 1
+    for i in range(3, rank - 1, -1):
+        kb *= fd[i]
+    if kb >= 10:
+        return True
+    if rank == 1 and fd[0] > 128:
+        return True
+    if rank == 2:
+        if fd[0] > 17 or fd[1] > 17:
+            return True
+        if fd[0] != fd[1] and fd[0] > 5:
+            return True
+    if rank == 3 and (fd[0] > 5 or fd[1] > 5 or fd[2] > 5):
+        return True
+    return False
+
+
+def conv_out_dims(sd, fd, kind: int, rank: int, expand: bool) -> tuple:
+    """Output dims, src/backend/cpu/convolve.cpp:44-58."""
+    od = [1, 1, 1, 1]
+    if expand:
+        for d in range(4):
+            if kind in (AF_BATCH_NONE, AF_BATCH_RHS):
+                od[d] = sd[d] + fd[d] - 1
+            else:
+                od[d] = sd[d] + fd[d] - 1 if d < rank else sd[d]
+    else:
+        od = list(sd)
+        if kind == AF_BATCH_RHS:
+            for i in range(rank, 4):
+                od[i] = fd[i]
+    return tuple(int(v) for v in od)
+
+
+# ------------------------------------------------------- spatial (restated)
+def convolve(signal, filt, rank: int, expand: bool) -> np.ndarray:
+    """Spatial convolution of rank 1/2/3 with the reference's batch rules.
+
+    Restates cpu::convolve<T, accT> (src/backend/cpu/convolve.cpp:38-66) and
+    kernel::convolve_nd (src/backend/cpu/kernel/convolve.hpp:132-211).  The
+    filter is converted to accT as src/api/c/convolve.cpp:46 (castArray) does.
+    Raises ValueError for the DIFF / UNSUPPORTED batch kinds.
+    """
+    x = as4(signal)
+    suf, acct = _SUFFIX[x.dtype]
+    sd, fd = dims4(x), dims4(filt)
+    kind = identify_batch_kind(rank, sd, fd)
+    if kind in (AF_BATCH_DIFF, AF_BATCH_UNSUPPORTED):
+        raise ValueError("unsupported batch kind")
+    od = conv_out_dims(sd, fd, kind, rank, expand)
+    xs = _flat(x, x.dtype)
+    fs = _flat(as4(filt), acct)
+    out = np.zeros(int(np.prod(od)), dtype=x.dtype)
+    getattr(_lib(), f"orc_convolve_nd_{suf}")(
+        _ptr(out), _ptr(_dimbuf(od)), _ptr(xs), _ptr(_dimbuf(sd)), _ptr(fs),
+        _ptr(_dimbuf(fd)), kind, rank, int(bool(expand)))
+    return out.reshape(od, order="F")
+
+
+def convolve2_sep(col_filter, row_filter, signal, expand: bool) -> np.ndarray:
+    """Separable 2-D convolution, src/backend/cpu/convolve.cpp:68-99 and
+    src/backend/cpu/kernel/convolve.hpp:213-289 (InT-typed temp).
+
+    The scalar/scalar case of src/api/c/convolve.cpp:57-68 is included."""
+    x = as4(signal)
+    suf, acct = _SUFFIX[x.dtype]
+    cf = np.asarray(col_filter, dtype=acct).ravel(order="F")
+    rf = np.asarray(row_filter, dtype=acct).ravel(order="F")
+    if cf.size == 1 and rf.size == 1:
+        prod = acct(cf[0]) * acct(rf[0])
+        return (x.astype(acct) * prod).astype(x.dtype)
+    sd = dims4(x)
+    od = list(sd)
+    if expand:
+        od[0] += cf.size - 1
+        od[1] += rf.size - 1
+    out = np.zeros(int(np.prod(od)), dtype=x.dtype)
+    xs = _flat(x, x.dtype)
+    rc = getattr(_lib(), f"orc_convolve2_sep_{suf}")(
+        _ptr(out), _ptr(_dimbuf(od)), _ptr(xs), _ptr(_dimbuf(sd)),
+        _ptr(np.ascontiguousarray(cf)), cf.size, _ptr(np.ascontiguousarray(rf)),
+        rf.size, int(bool(expand)))
+    if rc != 0:
+        raise MemoryError("oracle temp allocation failed")
+    return out.reshape(od, order="F")
+
+
+# ---------------------------------------------------------- NN (restated)
+def nn_out_size(s: int, f: int, stride: int, pad: int, dil: int) -> int:
+    """1 + (s + 2p - ((f-1)d + 1)) / stride with C++ truncating division,
+    src/backend/cpu/convolve.cpp:130-135."""
+    num = s + 2 * pad - ((f - 1) * dil + 1)
+    q = abs(num) // stride
+    return 1 + (q if num >= 0 else -q)
+
+
+def unwrap(x, wx, wy, sx, sy, px, py, dx, dy) -> np.ndarray:
+    """im2col, src/backend/cpu/unwrap.cpp:23-42 with retCols = false:
+    output dims [nx*ny, wx*wy, C, N]."""
+    x = as4(x)
+    suf = "f64" if x.dtype == np.float64 else "f32"
+    dt = np.float64 if suf == "f64" else np.float32
+    sd = dims4(x)
+    nx = nn_out_size(sd[0], wx, sx, px, dx)
+    ny = nn_out_size(sd[1], wy, sy, py, dy)
+    od = (nx * ny, wx * wy, sd[2], sd[3])
+    out = np.zeros(int(np.prod(od)), dtype=dt)
+    getattr(_lib(), f"orc_unwrap_{suf}")(
+        _ptr(out), _ptr(_flat(x, dt)), _ptr(_dimbuf(sd)),
+        wx, wy, sx, sy, px, py, dx, dy)
+    return out.reshape(od, order="F")
+
+
+def wrap_dilated(cols, ox, oy, wx, wy, sx, sy, px, py, dx, dy) -> np.ndarray:
+    """col2im, src/backend/cpu/wrap.cpp:60-75 (zero-initialised output,
+    serial scatter-add)."""
+    cols = as4(cols)
+    suf = "f64" if cols.dtype == np.float64 else "f32"
+    dt = np.float64 if suf == "f64" else np.float32
+    idm = dims4(cols)
+    od = (ox, oy, idm[2], idm[3])
+    out = np.zeros(int(np.prod(od)), dtype=dt)
+    getattr(_lib(), f"orc_wrap_dilated_{suf}")(
+        _ptr(out), _ptr(_dimbuf(od)), _ptr(_flat(cols, dt)), _ptr(_dimbuf(idm)),
+        wx, wy, sx, sy, px, py, dx, dy)
+    return out.reshape(od, order="F")
+
+
+def _fmod(a, shape):
+    """modDims: reinterpret in column-major order."""
+    return np.reshape(a, shape, order="F")
+
+
+def _flipped_collapsed_filter(w):
+    """flip(filter, {1,1,0,0}) then modDims [fw*fh*C, K]
+    (src/backend/cpu/convolve.cpp:147-149)."""
+    fd = dims4(w)
+    wf = as4(w)[::-1, ::-1, :, :]
+    return _fmod(wf, (fd[0] * fd[1] * fd[2], fd[3]))
+
+
+def _gemm_dtype(a):
+    # f16 runs its GEMM in f32 and rounds once (src/backend/cpu/blas.cpp:334-343)
+    return np.float32 if a.dtype in (np.float16, np.float32) else np.float64
+
+
+def conv2_nn(signal, filt, stride=(1, 1), padding=(0, 0), dilation=(1, 1)):
+    """convolve2_unwrap, src/backend/cpu/convolve.cpp:123-158.
+
+    signal [W,H,C,N], filter [fw,fh,C,K] -> [OW,OH,K,N]."""
+    x, w = as4(signal), as4(filt)
+    sd, fd = dims4(x), dims4(w)
+    gt = _gemm_dtype(x)
+    ow = nn_out_size(sd[0], fd[0], stride[0], padding[0], dilation[0])
+    oh = nn_out_size(sd[1], fd[1], stride[1], padding[1], dilation[1])
+    u = unwrap(x.astype(gt), fd[0], fd[1], stride[0], stride[1], padding[0],
+               padding[1], dilation[0], dilation[1])
+    u = np.transpose(u, (1, 2, 0, 3))                    # reorder(1,2,0,3)
+    ud = u.shape
+    u = _fmod(u, (ud[0] * ud[1], ud[2] * ud[3]))
+    wf = _flipped_collapsed_filter(w.astype(gt))
+    res = np.matmul(u.T, wf)                             # AF_MAT_TRANS, NONE
+    res = _fmod(res, (ow, oh, sd[3], fd[3]))
+    out = np.transpose(res, (0, 1, 3, 2))                # reorder(0,1,3,2)
+    return np.asfortranarray(out).astype(x.dtype)
+
+
+def _collapsed_gradient(dy):
+    cd = dims4(dy)
+    g = np.transpose(as4(dy), (0, 1, 3, 2))              # reorder(0,1,3,2)
+    return _fmod(g, (cd[0] * cd[1] * cd[3], cd[2]))
+
+
+def conv2_data_grad(dy, signal, filt, stride=(1, 1), padding=(0, 0),
+                    dilation=(1, 1)):
+    """conv2DataGradient, src/backend/cpu/convolve.cpp:179-211."""
+    x, w = as4(signal), as4(filt)
+    sd, fd = dims4(x), dims4(w)
+    gt = _gemm_dtype(x)
+    wf = _flipped_collapsed_filter(w.astype(gt))
+    g = _collapsed_gradient(as4(dy).astype(gt))
+    res = np.matmul(g, wf.T)                             # AF_MAT_NONE, TRANS
+    res = _fmod(res, (res.shape[0] // sd[3], sd[3], fd[0] * fd[1], sd[2]))
+    res = np.transpose(res, (0, 2, 3, 1))                # reorder(0,2,3,1)
+    if x.dtype == np.float16:
+        res = res.astype(np.float16).astype(np.float32)
+    out = wrap_dilated(np.asfortranarray(res), sd[0], sd[1], fd[0], fd[1],
+                       stride[0], stride[1], padding[0], padding[1],
+                       dilation[0], dilation[1])
+    return out.astype(x.dtype)
+
+
+def conv2_filter_grad(dy, signal, filt, stride=(1, 1), padding=(0, 0),
+                      dilation=(1, 1)):
+    """conv2FilterGradient, src/backend/cpu/convolve.cpp:213-243."""
+    x, w = as4(signal), as4(filt)
+    fd = dims4(w)
+    gt = _gemm_dtype(x)
+    u = unwrap(x.astype(gt), fd[0], fd[1], stride[0], stride[1], padding[0],
+               padding[1], dilation[0], dilation[1])
+    u = np.transpose(u, (1, 2, 0, 3))
+    ud = u.shape
+    u = _fmod(u, (ud[0] * ud[1], ud[2] * ud[3]))
+    g = _collapsed_gradient(as4(dy).astype(gt))
+    res = np.matmul(u, g)                                # AF_MAT_NONE, NONE
+    res = _fmod(res, fd)
+    return np.asfortranarray(res[::-1, ::-1, :, :]).astype(x.dtype)
+
+
+# ------------------------------------------------ FP64 brute-force forms
+def _tap_slice(xp, i, j, d, s, ow, oh):
+    """Strided view xp[i*d0 + ow*s0, j*d1 + oh*s1, :, :] for all (ow, oh)."""
+    a0 = i * d[0]
+    a1 = j * d[1]
+    return xp[a0:a0 + (ow - 1) * s[0] + 1:s[0], a1:a1 + (oh - 1) * s[1] + 1:s[1]]
+
+
+def _pad(x, p, extra=(0, 0)):
+    sd = x.shape
+    xp = np.zeros((sd[0] + 2 * p[0] + extra[0], sd[1] + 2 * p[1] + extra[1],
+                   sd[2], sd[3]), dtype=np.float64)
+    xp[p[0]:p[0] + sd[0], p[1]:p[1] + sd[1]] = x
+    return xp
+
+
+def conv2_nn_direct_f64(signal, filt, stride=(1, 1), padding=(0, 0),
+                        dilation=(1, 1)) -> np.ndarray:
+    """y[ow,oh,k,n] = sum_{c,i,j} x[ow*s0-p0+i*d0, oh*s1-p1+j*d1, c, n]
+    * w[fw-1-i, fh-1-j, c, k], zero outside x; evaluated in float64."""
+    x = as4(signal).astype(np.float64)
+    w = as4(filt).astype(np.float64)
+    sd, fd = x.shape, w.shape
+    ow = nn_out_size(sd[0], fd[0], stride[0], padding[0], dilation[0])
+    oh = nn_out_size(sd[1], fd[1], stride[1], padding[1], dilation[1])
+    xp = _pad(x, padding, extra=(max(0, stride[0]), max(0, stride[1])))
+    y = np.zeros((ow, oh, fd[3], sd[3]), dtype=np.float64)
+    for i in range(fd[0]):
+        for j in range(fd[1]):
+            xs = _tap_slice(xp, i, j, dilation, stride, ow, oh)  # [ow,oh,C,N]
+            y += np.einsum("abcn,ck->abkn", xs, w[fd[0] - 1 - i, fd[1] - 1 - j])
+    return y
+
+
+def conv2_data_grad_direct_f64(dy, signal, filt, stride=(1, 1),
+                               padding=(0, 0), dilation=(1, 1)) -> np.ndarray:
+    """Adjoint of conv2_nn_direct_f64 with respect to x, in float64."""
+    x = as4(signal)
+    w = as4(filt).astype(np.float64)
+    g = as4(dy).astype(np.float64)
+    sd, fd = x.shape, w.shape
+    ow, oh = g.shape[0], g.shape[1]
+    xp = np.zeros((sd[0] + 2 * padding[0] + stride[0],
+                   sd[1] + 2 * padding[1] + stride[1], sd[2], sd[3]))
+    for i in range(fd[0]):
+        for j in range(fd[1]):
+            t = np.einsum("abkn,ck->abcn", g, w[fd[0] - 1 - i, fd[1] - 1 - j])
+            a0, a1 = i * dilation[0], j * dilation[1]
+            xp[a0:a0 + (ow - 1) * stride[0] + 1:stride[0],
+               a1:a1 + (oh - 1) * stride[1] + 1:stride[1]] += t
+    return xp[padding[0]:padding[0] + sd[0], padding[1]:padding[1] + sd[1]]
+
+
+def conv2_filter_grad_direct_f64(dy, signal, filt, stride=(1, 1),
+                                 padding=(0, 0), dilation=(1, 1)) -> np.ndarray:
+    """Adjoint of conv2_nn_direct_f64 with respect to w, in float64."""
+    x = as4(signal).astype(np.float64)
+    g = as4(dy).astype(np.float64)
+    fd = dims4(filt)
+    ow, oh = g.shape[0], g.shape[1]
+    xp = _pad(x, padding, extra=(max(0, stride[0]), max(0, stride[1])))
+    dw = np.zeros(fd, dtype=np.float64)
+    for i in range(fd[0]):
+        for j in range(fd[1]):
+            xs = _tap_slice(xp, i, j, dilation, stride, ow, oh)
+            dw[fd[0] - 1 - i, fd[1] - 1 - j] = np.einsum("abcn,abkn->ck", xs, g)
+    return dw
+
+
+# ------------------------------------------------------------ filters
+def gaussian_kernel(rows: int, cols: int, sigma_r: float = 0.0,
+                    sigma_c: float = 0.0, dtype=np.float32) -> np.ndarray:
+    """src/api/c/gaussian_kernel.cpp:35-88: exp(-0.5*((i-(n-1)/2)/sigma)^2)
+    summed over the two axes, normalised to sum 1; sigma <= 0 means
+    0.25 * n.  Evaluated in ``dtype`` as the reference evaluates in T."""
+    t = np.zeros((rows, cols), dtype=dtype)
+    if cols > 1:
+        sig = sigma_c if sigma_c > 0 else 0.25 * cols
+        w = (np.arange(cols, dtype=dtype)[None, :] - dtype((cols - 1) / 2.0))
+        w = np.broadcast_to(w / dtype(sig), (rows, cols))
+        t = t + w * w
+    if rows > 1:
+        sig = sigma_r if sigma_r > 0 else 0.25 * rows
+        w = (np.arange(rows, dtype=dtype)[:, None] - dtype((rows - 1) / 2.0))
+        w = np.broadcast_to(w / dtype(sig), (rows, cols))
+        t = w * w + t
+    t = np.exp(-(dtype(0.5) * t)).astype(dtype)
+    return (t / t.sum(dtype=dtype)).astype(dtype)

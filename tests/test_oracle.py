@@ -1,0 +1,217 @@
+"""Pins the CPU oracle (test infrastructure) before it is trusted.
+
+Each check is chosen so that a plausible restatement mistake (a dropped tap,
+a sign or index error, a transposed operand, a wrong centring offset) fails:
+  * known answers printed in the reference's tests (test/convolve.cpp:522-620,
+    1170-1185), stored in tests/golden/known_answers.json;
+  * scipy.signal.convolve (textbook full convolution) for every rank/mode;
+  * batch-kind equivalences (LHS/RHS/SAME == loops of NONE);
+  * separable == outer-product 2-D (examples/getting_started/convolve.cpp:29-52);
+  * im2col+GEMM NN path == independent FP64 closed form, == spatial DEFAULT
+    (test/convolve.cpp:828-894), adjoint identities, central differences;
+  * the Gaussian builder's closed form.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.signal as ss
+
+import oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def golden():
+    with open(os.path.join(HERE, "golden", "known_answers.json")) as f:
+        return json.load(f)
+
+
+def test_known_answer_1d(golden):
+    g = golden["convolve_1d"]
+    got = oracle.convolve(np.float32(g["a"]), np.float32(g["b"]), 1, False).ravel()
+    np.testing.assert_allclose(got, g["c"], atol=1.5e-4)
+
+
+def test_known_answer_2d(golden):
+    g = golden["convolve_2d"]
+    got = oracle.convolve(np.full(g["signal_dims"], g["signal_value"], np.float32),
+                          np.full(g["filter_dims"], g["filter_value"], np.float32), 2, False)
+    np.testing.assert_array_equal(got[:, :, 0, 0], np.array(g["output"]))
+
+
+def test_known_answer_3d(golden):
+    g = golden["convolve_3d"]
+    got = oracle.convolve(np.full(g["signal_dims"], g["signal_value"], np.float32),
+                          np.full(g["filter_dims"], g["filter_value"], np.float32), 3, False)
+    # output slices as printed, indexed [k][i][j]
+    exp = np.array(g["output_slices"])
+    for k in range(4):
+        np.testing.assert_array_equal(got[:, :, k, 0], exp[k])
+
+
+def _scipy_conv(x, w, expand):
+    full = ss.convolve(x.astype(np.float64), w.astype(np.float64), mode="full", method="direct")
+    if expand:
+        return full
+    sl = tuple(slice(f // 2, f // 2 + s) for s, f in zip(x.shape, w.shape))
+    return full[sl]
+
+
+@pytest.mark.parametrize("sd,fd", [((31,), (4,)), ((31,), (7,)), ((13, 9), (4, 3)),
+                                   ((13, 9), (5, 5)), ((7, 8, 9), (2, 3, 4)),
+                                   ((7, 8, 9), (3, 3, 3))])
+@pytest.mark.parametrize("expand", [False, True])
+def test_spatial_vs_scipy(rng, sd, fd, expand):
+    x = rng.random(sd)
+    w = rng.uniform(-1, 1, fd)
+    got = oracle.convolve(x, w, len(sd), expand)
+    ref = _scipy_conv(x, w, expand)
+    assert got.shape[:len(sd)] == ref.shape
+    np.testing.assert_allclose(got.reshape(ref.shape, order="F"), ref, rtol=1e-12, atol=1e-12)
+
+
+def test_batch_kinds_are_loops_of_none(rng):
+    x = rng.random((9, 8, 3))
+    w = rng.uniform(-1, 1, (3, 2, 3))
+    assert oracle.identify_batch_kind(2, (9, 8, 3, 1), (3, 2, 3, 1)) == oracle.AF_BATCH_SAME
+    assert oracle.identify_batch_kind(2, (9, 8, 3, 1), (3, 2, 1, 1)) == oracle.AF_BATCH_LHS
+    assert oracle.identify_batch_kind(2, (9, 8, 1, 1), (3, 2, 3, 1)) == oracle.AF_BATCH_RHS
+    assert oracle.identify_batch_kind(2, (9, 8, 3, 1), (3, 2, 1, 2)) == oracle.AF_BATCH_DIFF
+    assert oracle.identify_batch_kind(2, (9, 8, 3, 1), (3, 2, 2, 1)) == oracle.AF_BATCH_UNSUPPORTED
+    for expand in (False, True):
+        same = oracle.convolve(x, w, 2, expand)
+        lhs = oracle.convolve(x, w[:, :, 0], 2, expand)
+        rhs = oracle.convolve(x[:, :, 0], w, 2, expand)
+        for k in range(3):
+            np.testing.assert_array_equal(same[:, :, k, 0],
+                                          oracle.convolve(x[:, :, k], w[:, :, k], 2, expand)[:, :, 0, 0])
+            np.testing.assert_array_equal(lhs[:, :, k, 0],
+                                          oracle.convolve(x[:, :, k], w[:, :, 0], 2, expand)[:, :, 0, 0])
+            np.testing.assert_array_equal(rhs[:, :, k, 0],
+                                          oracle.convolve(x[:, :, 0], w[:, :, k], 2, expand)[:, :, 0, 0])
+
+
+def test_out_dims_rules():
+    N, L, R, S = (oracle.AF_BATCH_NONE, oracle.AF_BATCH_LHS, oracle.AF_BATCH_RHS,
+                  oracle.AF_BATCH_SAME)
+    assert oracle.conv_out_dims((10, 11, 1, 1), (3, 4, 1, 1), N, 2, True) == (12, 14, 1, 1)
+    assert oracle.conv_out_dims((10, 11, 5, 1), (3, 4, 1, 1), L, 2, True) == (12, 14, 5, 1)
+    assert oracle.conv_out_dims((10, 11, 1, 1), (3, 4, 5, 1), R, 2, True) == (12, 14, 5, 1)
+    assert oracle.conv_out_dims((10, 11, 1, 1), (3, 4, 5, 1), R, 2, False) == (10, 11, 5, 1)
+    assert oracle.conv_out_dims((10, 11, 5, 2), (3, 4, 5, 2), S, 2, True) == (12, 14, 5, 2)
+    assert oracle.ndims((1, 1, 1, 1)) == 1 and oracle.ndims((0, 3, 1, 1)) == 0
+    assert oracle.ndims((1, 3, 1, 1)) == 2 and oracle.ndims((4, 1, 1, 2)) == 4
+
+
+def test_is_freq_domain_rules():
+    assert oracle.is_freq_domain(2, (40, 40, 1, 1), (7, 3, 1, 1), 0)
+    assert not oracle.is_freq_domain(2, (40, 40, 1, 1), (5, 3, 1, 1), 0)
+    assert oracle.is_freq_domain(2, (40, 40, 1, 1), (19, 19, 1, 1), 0)
+    assert oracle.is_freq_domain(2, (40, 40, 1, 1), (3, 3, 10, 1), 0)
+    assert not oracle.is_freq_domain(2, (40, 40, 1, 1), (3, 3, 9, 1), 0)
+    assert oracle.is_freq_domain(1, (400, 1, 1, 1), (129, 1, 1, 1), 0)
+    assert not oracle.is_freq_domain(2, (40, 40, 1, 1), (19, 19, 1, 1), 1)
+
+
+@pytest.mark.parametrize("expand", [False, True])
+def test_separable_is_outer_product(rng, expand):
+    x = rng.random((23, 19, 2)).astype(np.float32)
+    cf = rng.uniform(-1, 1, 5).astype(np.float32)
+    rf = rng.uniform(-1, 1, 4).astype(np.float32)
+    got = oracle.convolve2_sep(cf, rf, x, expand)
+    full = oracle.convolve(x.astype(np.float64), np.outer(cf, rf).astype(np.float64), 2, expand)
+    assert got.shape == full.shape
+    assert oracle.rel_l2(got, full) <= 1e-6
+
+
+# (x dims, w dims, stride, pad, dilation)
+NN = [((10, 10, 1, 1), (3, 3, 1, 1), (1, 1), (1, 1), (1, 1)),
+      ((10, 10, 1, 1), (3, 5, 1, 1), (3, 1), (1, 1), (2, 1)),
+      ((8, 10, 3, 2), (3, 3, 3, 4), (2, 2), (1, 1), (1, 1)),
+      ((9, 7, 5, 2), (2, 4, 5, 3), (1, 2), (0, 2), (2, 1))]
+
+
+@pytest.mark.parametrize("g", NN)
+def test_nn_restatement_vs_closed_form(rng, g):
+    x = rng.standard_normal(g[0])
+    w = rng.standard_normal(g[1])
+    y = oracle.conv2_nn(x, w, *g[2:])
+    y64 = oracle.conv2_nn_direct_f64(x, w, *g[2:])
+    assert y.shape == y64.shape
+    assert oracle.rel_l2(y, y64) <= 1e-13
+    dy = rng.standard_normal(y.shape)
+    assert oracle.rel_l2(oracle.conv2_data_grad(dy, x, w, *g[2:]),
+                         oracle.conv2_data_grad_direct_f64(dy, x, w, *g[2:])) <= 1e-13
+    assert oracle.rel_l2(oracle.conv2_filter_grad(dy, x, w, *g[2:]),
+                         oracle.conv2_filter_grad_direct_f64(dy, x, w, *g[2:])) <= 1e-13
+    # adjoint identities <conv(x,w), dy> = <x, dX(dy)> = <w, dW(dy)>
+    lhs = float(np.sum(y64 * dy))
+    assert abs(lhs - float(np.sum(x * oracle.conv2_data_grad_direct_f64(dy, x, w, *g[2:])))) \
+        <= 1e-10 * max(1, abs(lhs))
+    assert abs(lhs - float(np.sum(w * oracle.conv2_filter_grad_direct_f64(dy, x, w, *g[2:])))) \
+        <= 1e-10 * max(1, abs(lhs))
+
+
+def test_nn_matches_spatial_default(rng):
+    # Conv2ConsistencyTest invariant, test/convolve.cpp:828-894
+    for fs in (3, 5, 7, 9):
+        x = rng.standard_normal((19, 17))
+        w = rng.standard_normal((fs, fs))
+        nn = oracle.conv2_nn(x, w, (1, 1), (fs // 2, fs // 2), (1, 1))
+        sp = oracle.convolve(x, w, 2, False)
+        np.testing.assert_allclose(nn, sp, atol=1e-12)
+
+
+def test_zero_padding_issue2817():
+    s = np.ones((5, 5), np.float32)
+    f = np.full((3, 3), 1 / 9, np.float32)
+    y = oracle.conv2_nn(s, f, (1, 1), (0, 0), (1, 1))
+    assert y.shape == (3, 3, 1, 1) and np.sum(np.abs(y - 1)) < 1e-5
+    g = np.full((3, 3), 1 / 9, np.float32)
+    dw = oracle.conv2_filter_grad(g, s, f, (1, 1), (0, 0), (1, 1))
+    assert np.sum(np.abs(dw - y)) < 1e-5
+
+
+def test_nn_finite_differences(rng):
+    st, pd, dl = (2, 2), (1, 1), (1, 1)
+    x = rng.standard_normal((6, 6, 2, 2))
+    w = rng.standard_normal((3, 3, 2, 3))
+    g = rng.standard_normal(oracle.conv2_nn_direct_f64(x, w, st, pd, dl).shape)
+    h = 1e-6
+    L = lambda xx, ww: float(np.sum(oracle.conv2_nn_direct_f64(xx, ww, st, pd, dl) * g))  # noqa
+    dx = oracle.conv2_data_grad(g, x, w, st, pd, dl)
+    for idx in [(0, 0, 0, 0), (3, 2, 1, 1), (5, 5, 0, 1), (1, 4, 1, 0)]:
+        xp, xm = x.copy(), x.copy()
+        xp[idx] += h
+        xm[idx] -= h
+        assert abs((L(xp, w) - L(xm, w)) / (2 * h) - dx[idx]) < 1e-6
+    dw = oracle.conv2_filter_grad(g, x, w, st, pd, dl)
+    for idx in [(0, 0, 0, 0), (2, 1, 1, 2), (1, 2, 0, 1)]:
+        wp, wm = w.copy(), w.copy()
+        wp[idx] += h
+        wm[idx] -= h
+        assert abs((L(x, wp) - L(x, wm)) / (2 * h) - dw[idx]) < 1e-6
+
+
+def test_nn_out_size_truncates():
+    assert oracle.nn_out_size(10, 3, 1, 1, 1) == 10
+    assert oracle.nn_out_size(10, 3, 3, 1, 1) == 4
+    assert oracle.nn_out_size(10, 5, 5, 1, 1) == 2
+    # (2 - 5) / 2 truncates toward zero to -1, so the size is 0
+    assert oracle.nn_out_size(2, 5, 2, 0, 1) == 0
+
+
+def test_gaussian_kernel_closed_form():
+    # src/api/c/gaussian_kernel.cpp:35-88
+    g = oracle.gaussian_kernel(17, 1, 3.0, dtype=np.float64).ravel()
+    assert abs(g.sum() - 1) < 1e-15
+    np.testing.assert_allclose(g, g[::-1], rtol=0, atol=0)
+    i = np.arange(17)
+    np.testing.assert_allclose(g / g[8], np.exp(-0.5 * ((i - 8) / 3.0) ** 2), rtol=1e-14)
+    d = oracle.gaussian_kernel(5, 5, dtype=np.float64)  # default sigma 0.25 * n
+    r = np.arange(5) - 2.0
+    e = np.exp(-0.5 * ((r[:, None] / 1.25) ** 2 + (r[None, :] / 1.25) ** 2))
+    np.testing.assert_allclose(d, e / e.sum(), rtol=1e-14)

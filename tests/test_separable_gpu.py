@@ -1,0 +1,80 @@
+"""af_convolve2_sep on the GPU against the oracle (two-pass restatement with
+the reference's T-typed temp) and against the outer-product 2-D convolution
+(examples/getting_started/convolve.cpp:29-52)."""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    ((3, 4, 2), 2, 3),          # separable_conv2d_same_rectangle_batch dims
+    ((64, 64), 5, 5),
+    ((100, 37, 3), 17, 17),
+    ((130, 129), 4, 7),
+    ((70, 90, 2, 2), 33, 2),
+    ((50, 20), 2, 9),
+    ((31, 33), 40, 3),          # beyond the fused kernel's 33 taps
+    ((45, 21), 3, 35),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"s{'x'.join(map(str, c[0]))}_c{c[1]}_r{c[2]}")
+@pytest.mark.parametrize("mode", [0, 1], ids=["default", "expand"])
+def test_f32(af, rng, case, mode):
+    sd, cl, rl = case
+    x = rng.random(sd).astype(np.float32)
+    cf = rng.uniform(-1, 1, cl).astype(np.float32)
+    rf = rng.uniform(-1, 1, rl).astype(np.float32)
+    got = af.convolve_sep(cf, rf, x, mode).to_numpy()
+    ref = oracle.convolve2_sep(cf, rf, x, mode == 1)
+    assert got.shape == ref.shape
+    assert oracle.rel_l2(got, ref) <= 1e-5
+    # equals the 2-D convolution with the outer-product filter
+    full = oracle.convolve(x.astype(np.float64), np.outer(cf, rf).astype(np.float64), 2, mode == 1)
+    assert oracle.rel_l2(got, full) <= 1e-5
+
+
+@pytest.mark.parametrize("mode", [0, 1], ids=["default", "expand"])
+def test_integer_valued_bit_exact(af, rng, mode):
+    x = rng.integers(0, 8, (40, 33, 2)).astype(np.float32)
+    cf = rng.integers(-2, 3, 5).astype(np.float32)
+    rf = rng.integers(-2, 3, 4).astype(np.float32)
+    got = af.convolve_sep(cf, rf, x, mode).to_numpy()
+    np.testing.assert_array_equal(got, oracle.convolve2_sep(cf, rf, x, mode == 1))
+
+
+def test_f64_and_int(af, rng):
+    x = rng.random((40, 30))
+    cf, rf = rng.uniform(-1, 1, 5), rng.uniform(-1, 1, 7)
+    got = af.convolve_sep(cf, rf, x, 0).to_numpy()
+    assert oracle.rel_l2(got, oracle.convolve2_sep(cf, rf, x, False)) <= 1e-13
+    xi = rng.integers(0, 100, (40, 30)).astype(np.int32)
+    got = af.convolve_sep(cf.astype(np.float32), rf.astype(np.float32), xi, 1).to_numpy()
+    ref = oracle.convolve2_sep(cf.astype(np.float32), rf.astype(np.float32), xi, True)
+    assert got.dtype == np.int32 and got.shape == ref.shape
+    assert np.max(np.abs(got.astype(np.int64) - ref)) <= 1
+
+
+def test_gaussian_17_sigma3(af, rng):
+    g = oracle.gaussian_kernel(17, 1, 3.0).ravel()
+    x = rng.random((256, 200, 2)).astype(np.float32)
+    got = af.convolve_sep(g, g, x, 0).to_numpy()
+    assert oracle.rel_l2(got, oracle.convolve2_sep(g, g, x, False)) <= 1e-5
+
+
+def test_length_one_filter_is_not_a_vector(af, rng):
+    # isVector needs exactly one non-1 dim (src/backend/common/ArrayInfo.cpp:122-130):
+    # a length-1 filter beside a longer one is an argument error
+    x = rng.random((20, 20)).astype(np.float32)
+    with pytest.raises(af.AfError) as e:
+        af.convolve_sep(np.float32([2.0]), np.ones(5, np.float32), x)
+    assert e.value.code == af.AF_ERR_ARG
+
+
+def test_scalar_filters(af, rng):
+    # both filters scalar: elementwise T(accT(x) * c * r) (convolve.cpp:57-68)
+    x = rng.random((9, 7)).astype(np.float32)
+    got = af.convolve_sep(np.float32([0.5]), np.float32([3.0]), x, 0).to_numpy()
+    np.testing.assert_array_equal(got[:, :, 0, 0], x * np.float32(1.5))
