@@ -6,6 +6,7 @@
 // output dims, allocates from the stream-ordered pool and queues the gfx950
 // kernels through the kernel C ABI (include/afhip_kernels.h).
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "../../include/af/array.h"
@@ -194,6 +195,24 @@ static void check_nn_geom(const NNGeom &g) {
 // f64 inputs run the direct FP64 kernels (afhip_k_conv2_nn_f64); the MFMA
 // implicit-GEMM kernels take FP32 and 16-bit operands only.
 
+// The patch-staged kernels serve the forward pass and the stride-1 data
+// gradient when the input patch fits in LDS, the row-staged kernel the filter
+// gradient of 16-bit operands; AFHIP_NN_GATHER=1 forces the gather kernels
+// (the tests use it to cover both paths).
+static bool run_patch(int pass, const ArrayImpl &in, const ArrayImpl &w, af_dtype out_type,
+                      int precision, void *out, const NNGeom &g) {
+    const char *env = std::getenv("AFHIP_NN_GATHER");
+    if (env && env[0] == '1') return false;
+    const long long need = afhip_k_conv2_nn_patch_workspace(pass, in.type, precision, g.x.d, g.w.d,
+                                                            g.y.d, g.st, g.pd, g.dl);
+    if (need <= 0) return false;
+    auto ws = scratch((size_t)need);
+    check_k(afhip_k_conv2_nn_patch(pass, in.type, out_type, precision, out, in.data(), w.data(), g.x.d,
+                                   g.w.d, g.y.d, g.st, g.pd, g.dl, ws->ptr, need, stream()),
+            pass == 0 ? "convolve2_nn" : "convolve2_gradient_nn");
+    return true;
+}
+
 static af_dtype nn_out_type(af_dtype in, af_dtype requested) {
     if ((int)requested == -1) return in;
     return requested;
@@ -210,6 +229,7 @@ static af_array conv2_nn(const ArrayImpl &x, const ArrayImpl &w, const NNGeom &g
                 "convolve2_nn");
         return out;
     }
+    if (run_patch(0, x, w, out_type, precision, get(out).data(), g)) return out;
     check_k(afhip_k_conv2_nn_fwd(x.type, out_type, precision, get(out).data(), x.data(), w.data(), g.x.d,
                                  g.w.d, g.y.d, g.st, g.pd, g.dl, stream()),
             "convolve2_nn");
@@ -232,6 +252,7 @@ static af_array conv2_data_grad(const ArrayImpl &dy, const ArrayImpl &x, const A
                 "convolve2_gradient_nn");
         return out;
     }
+    if (run_patch(1, dy, w, out_type, precision, o, g)) return out;
     check_k(afhip_k_conv2_nn_dgrad(x.type, out_type, precision, o, dy.data(), w.data(), g.x.d, g.w.d,
                                    g.y.d, g.st, g.pd, g.dl, stream()),
             "convolve2_gradient_nn");
@@ -253,6 +274,18 @@ static af_array conv2_filter_grad(const ArrayImpl &dy, const ArrayImpl &x, const
                                      stream()),
                 "convolve2_gradient_nn");
         return out;
+    }
+    const char *env = std::getenv("AFHIP_NN_GATHER");
+    if (!(env && env[0] == '1')) {
+        const long long rb = afhip_k_conv2_nn_wgrad_rows_workspace(x.type, precision, g.x.d, g.w.d, g.y.d,
+                                                                   g.st, g.pd, g.dl);
+        if (rb > 0) {
+            auto ws = scratch((size_t)rb);
+            check_k(afhip_k_conv2_nn_wgrad_rows(x.type, out_type, precision, o, x.data(), dy.data(), g.x.d,
+                                                g.w.d, g.y.d, g.st, g.pd, g.dl, ws->ptr, rb, stream()),
+                    "convolve2_gradient_nn");
+            return out;
+        }
     }
     const long long wsb = afhip_k_conv2_nn_wgrad_workspace(g.x.d, g.w.d, g.y.d, g.st, g.pd, g.dl);
     std::shared_ptr<Buffer> ws;
@@ -450,7 +483,8 @@ static af_err nn_gradient(af_array *out, const af_array incoming_gradient,
                           const af_array convolved_output, const unsigned stride_dims,
                           const dim_t *strides, const unsigned padding_dims, const dim_t *paddings,
                           const unsigned dilation_dims, const dim_t *dilations,
-                          af_conv_gradient_type grad_type, int precision, af_dtype out_type) {
+                          af_conv_gradient_type grad_type, int precision, af_dtype out_type,
+                          bool strict_types) {
     try {
         const ArrayImpl &g = get(incoming_gradient);
         const ArrayImpl &s = get(original_signal);
@@ -468,9 +502,14 @@ static af_err nn_gradient(af_array *out, const af_array incoming_gradient,
         ARG_ASSERT(3, sn > 0 && sn <= 2);
         ARG_ASSERT(5, pn >= 0 && pn <= 2);
         ARG_ASSERT(7, dn > 0 && dn <= 2);
-        const af_dtype t = o.type;
+        // The reference takes T from convolved_output and requires all four
+        // arrays to be T (src/api/c/convolve.cpp:452, handle.hpp:53-54). The
+        // extension entry point reads only the dims of convolved_output (it
+        // may hold the FP32 output of a BF16 forward), so T comes from the
+        // incoming gradient there.
+        const af_dtype t = strict_types ? o.type : g.type;
         if (t != f32 && t != f64 && t != f16 && t != afhip_bf16) TYPE_ERROR(1, t);
-        if (g.type != t || s.type != t || f.type != t)
+        if (g.type != t || s.type != t || f.type != t || (strict_types && o.type != t))
             AFHIP_THROW(AF_ERR_INTERNAL, "Invalid type for input array.");
         const af_dtype ot = detail::nn_out_type(t, out_type);
         if (t == f64) {
@@ -500,7 +539,7 @@ AFAPI af_err af_convolve2_gradient_nn(af_array *out, const af_array incoming_gra
                                       const dim_t *dilations, af_conv_gradient_type grad_type) {
     return nn_gradient(out, incoming_gradient, original_signal, original_filter, convolved_output,
                        stride_dims, strides, padding_dims, paddings, dilation_dims, dilations,
-                       grad_type, AFHIP_PRECISION_DEFAULT, (af_dtype)-1);
+                       grad_type, AFHIP_PRECISION_DEFAULT, (af_dtype)-1, true);
 }
 
 AFAPI af_err afhip_convolve2_gradient_nn_ex(
@@ -511,7 +550,7 @@ AFAPI af_err afhip_convolve2_gradient_nn_ex(
     afhip_precision precision, af_dtype out_type) {
     return nn_gradient(out, incoming_gradient, original_signal, original_filter, convolved_output,
                        stride_dims, strides, padding_dims, paddings, dilation_dims, dilations,
-                       grad_type, (int)precision, out_type);
+                       grad_type, (int)precision, out_type, false);
 }
 
 // include/af/data.h:515

@@ -1,0 +1,514 @@
+// convolve2_nn filter gradient on the gfx950 matrix cores, 16-bit operands
+// (BF16 or FP16) with FP32 accumulation:
+//   dW[fw-1-i, fh-1-j, c, k] = sum_{n, oh, ow} x[n][c][oh*s1-p1+j*d1][ow*s0-p0+i] * dy[n][k][oh][ow]
+//
+// GEMM view per tap (i, j): D[k][c] += A[k][pixels] * B[pixels][c], with the
+// reduction over output pixels. Both operands stay in their natural NCHW row
+// layout: A is a block of dy rows, B the rows of x the block touches. A lane's
+// 8 reduction values are 8 consecutive output columns of one row (columns are
+// padded to a multiple of 8 with dy = 0), so its A fragment is one aligned
+// 16-byte LDS read, and its B fragment is 8 x values at column stride S0
+// starting at an offset that depends only on the tap. B fragments for all FW
+// taps of a filter row come from one window of dword reads, realigned with
+// v_alignbyte (S0 = 1) or v_perm (S0 = 2): the window is read once and serves
+// FW taps x 4 filter tiles of MFMAs.
+//
+// Workgroup: 4 waves; tile = 64 filters x 64 channels x all taps; wave w owns
+// channels 16w..16w+15 and all 64 filters (4 MFMA row tiles), so its
+// accumulators are 4 x FW*fh tiles of 16x16. The reduction over N*OH*OW is cut
+// into blocks of R_oh output rows of one image, staged in LDS; the blocks are
+// split across workgroups, whose FP32 partial tiles a second kernel sums in a
+// fixed order (deterministic).
+#include "kcommon.hpp"
+#include "../../../include/afhip_kernels.h"
+
+namespace afhip {
+
+typedef __bf16 wbf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 wf16x8 __attribute__((ext_vector_type(8)));
+typedef float wf32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t wu32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int WG_BK = 64, WG_BC = 64, WG_THREADS = 256;
+constexpr size_t WG_LDS_LIMIT = 78 * 1024;
+
+struct WGParams {
+    int W, H, C, N, OW, OH, K;
+    int fh, s1, p0, p1, d1;
+    int OWp, segrow;     // padded output width, 8-pixel segments per row
+    int R_oh, nseg, steps, dyp;  // rows per block, segments, MFMA steps, dy LDS pitch
+    int PR, PADL, PWpad; // patch rows, left pad (multiple of 8), patch pitch
+    int units_per_img, total_units, nsplit, upsplit;
+    int ktiles, ctiles, RS;
+    int is_f16;
+};
+
+constexpr int WG_MAXD = 8;   // dy chunks (8 values) per thread per block
+constexpr int WG_MAXX = 16;  // x chunks per thread per block
+
+// Register staging of one block: fetch() issues every global load of the
+// block (independent, so their latencies overlap each other and the MFMA work
+// of the block being computed), commit() stores them to LDS.
+struct BlockRegs {
+    uint4 d[WG_MAXD];
+    uint4 x[WG_MAXX];
+};
+
+__device__ inline uint4 pack8(const uint16_t *h) {
+    return make_uint4(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16),
+                      h[4] | ((uint32_t)h[5] << 16), h[6] | ((uint32_t)h[7] << 16));
+}
+
+// Aligned rows only (W and OW multiples of 8): every 8-value chunk of a row is
+// a single aligned 16-byte load or lies wholly outside the row.
+__device__ inline void fetch_block(BlockRegs &R, const uint16_t *__restrict__ x,
+                                   const uint16_t *__restrict__ dy, const WGParams &p, int n, int oh0,
+                                   int kt, int ct) {
+    const int tid = threadIdx.x;
+    const long long HW = (long long)p.H * p.W, OHW = (long long)p.OH * p.OW;
+    const int segs = p.steps * 4;
+#pragma unroll
+    for (int q = 0; q < WG_MAXD; ++q) {
+        const int e = tid + q * WG_THREADS;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        const int m = e / segs;
+        const int seg = e - m * segs;
+        const int k = kt * WG_BK + m;
+        if (m < WG_BK && seg < p.nseg && k < p.K) {
+            const int oh = oh0 + seg / p.segrow;
+            const int ow0 = (seg % p.segrow) * 8;
+            if (oh < p.OH) {
+                const uint16_t *src = dy + ((long long)n * p.K + k) * OHW + (long long)oh * p.OW + ow0;
+                v = *reinterpret_cast<const uint4 *>(src);
+            }
+        }
+        R.d[q] = v;
+    }
+    const int qrow = p.PWpad / 8;
+    const int xchunks = WG_BC * p.PR * qrow;
+#pragma unroll
+    for (int q = 0; q < WG_MAXX; ++q) {
+        const int e = tid + q * WG_THREADS;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (e < xchunks) {
+            const int cr = e / qrow;
+            const int qq = e - cr * qrow;
+            const int c = cr / p.PR, r = cr - (cr / p.PR) * p.PR;
+            const int ch = ct * WG_BC + c;
+            const int ih = oh0 * p.s1 - p.p1 + r;
+            const int iw0 = qq * 8 - p.PADL;
+            if (ch < p.C && ih >= 0 && ih < p.H && iw0 + 8 > 0 && iw0 < p.W) {
+                const uint16_t *src = x + ((long long)n * p.C + ch) * HW + (long long)ih * p.W;
+                v = *reinterpret_cast<const uint4 *>(src + iw0);
+            }
+        }
+        R.x[q] = v;
+    }
+}
+
+__device__ inline void commit_block(const BlockRegs &R, uint16_t *lds_dy, uint16_t *lds_x,
+                                    const WGParams &p) {
+    const int tid = threadIdx.x;
+    const int segs = p.steps * 4;
+#pragma unroll
+    for (int q = 0; q < WG_MAXD; ++q) {
+        const int e = tid + q * WG_THREADS;
+        const int m = e / segs;
+        const int seg = e - m * segs;
+        if (m < WG_BK) *reinterpret_cast<uint4 *>(lds_dy + m * p.dyp + seg * 8) = R.d[q];
+    }
+    const int qrow = p.PWpad / 8;
+    const int xchunks = WG_BC * p.PR * qrow;
+#pragma unroll
+    for (int q = 0; q < WG_MAXX; ++q) {
+        const int e = tid + q * WG_THREADS;
+        if (e < xchunks) {
+            const int cr = e / qrow;
+            const int qq = e - cr * qrow;
+            *reinterpret_cast<uint4 *>(lds_x + cr * p.PWpad + qq * 8) = R.x[q];
+        }
+    }
+}
+
+// Unaligned rows (W or OW not a multiple of 8): element loads straight to
+// LDS, four chunks in flight at a time. Not overlapped with the MFMA work.
+__device__ inline void load_direct(const uint16_t *__restrict__ x, const uint16_t *__restrict__ dy,
+                                   uint16_t *lds_dy, uint16_t *lds_x, const WGParams &p, int n, int oh0,
+                                   int kt, int ct) {
+    const int tid = threadIdx.x;
+    const long long HW = (long long)p.H * p.W, OHW = (long long)p.OH * p.OW;
+    const int segs = p.steps * 4;
+    const int dchunks = WG_BK * segs;
+    for (int e0 = 0; e0 < dchunks; e0 += 4 * WG_THREADS) {
+        uint4 v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = e0 + q * WG_THREADS + tid;
+            const int m = e / segs, seg = e - (e / segs) * segs;
+            const int k = kt * WG_BK + m;
+            uint16_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (e < dchunks && seg < p.nseg && k < p.K) {
+                const int oh = oh0 + seg / p.segrow;
+                const int ow0 = (seg % p.segrow) * 8;
+                if (oh < p.OH) {
+                    const uint16_t *src = dy + ((long long)n * p.K + k) * OHW + (long long)oh * p.OW + ow0;
+#pragma unroll
+                    for (int t = 0; t < 8; ++t)
+                        if (ow0 + t < p.OW) h[t] = src[t];
+                }
+            }
+            v[q] = pack8(h);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = e0 + q * WG_THREADS + tid;
+            if (e < dchunks) {
+                const int m = e / segs, seg = e - (e / segs) * segs;
+                *reinterpret_cast<uint4 *>(lds_dy + m * p.dyp + seg * 8) = v[q];
+            }
+        }
+    }
+    const int qrow = p.PWpad / 8;
+    const int xchunks = WG_BC * p.PR * qrow;
+    for (int e0 = 0; e0 < xchunks; e0 += 4 * WG_THREADS) {
+        uint4 v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = e0 + q * WG_THREADS + tid;
+            const int cr = e / qrow, qq = e - (e / qrow) * qrow;
+            const int c = cr / p.PR, r = cr - (cr / p.PR) * p.PR;
+            const int ch = ct * WG_BC + c;
+            const int ih = oh0 * p.s1 - p.p1 + r;
+            const int iw0 = qq * 8 - p.PADL;
+            uint16_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (e < xchunks && ch < p.C && ih >= 0 && ih < p.H) {
+                const uint16_t *src = x + ((long long)n * p.C + ch) * HW + (long long)ih * p.W;
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    const int iw = iw0 + t;
+                    if (iw >= 0 && iw < p.W) h[t] = src[iw];
+                }
+            }
+            v[q] = pack8(h);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = e0 + q * WG_THREADS + tid;
+            if (e < xchunks) {
+                const int cr = e / qrow, qq = e - (e / qrow) * qrow;
+                *reinterpret_cast<uint4 *>(lds_x + cr * p.PWpad + qq * 8) = v[q];
+            }
+        }
+    }
+}
+
+// B fragments of the FW taps of one filter row from a window of dword reads.
+// rowp: the x row in LDS; e0: element index of the first value for tap i=0
+// (its parity `odd` is the same for every lane).
+template <int FW, int S0>
+__device__ inline void b_frags(const uint16_t *rowp, int e0, wu32x4 (&bf)[FW]) {
+    constexpr int ND = S0 == 1 ? (8 + FW + 1) / 2 + 1 : (16 + FW + 1) / 2 + 1;
+    const uint32_t *dw = reinterpret_cast<const uint32_t *>(rowp) + (e0 >> 1);
+    uint32_t d[ND];
+#pragma unroll
+    for (int q = 0; q < ND; ++q) d[q] = dw[q];
+    // `odd` (the parity of e0) is the same for every lane: branch on it so
+    // that every register index below is a compile-time constant
+    if ((e0 & 1) == 0) {
+#pragma unroll
+        for (int i = 0; i < FW; ++i) {
+            if constexpr (S0 == 1) {
+                // out dword m = elements (i + 2m, i + 2m + 1)
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+                    bf[i][m] = (i & 1) ? __builtin_amdgcn_alignbyte(d[(i + 1) / 2 + m], d[(i - 1) / 2 + m], 2)
+                                       : d[i / 2 + m];
+            } else {
+                // out dword m = elements (i + 4m, i + 4m + 2): one half of
+                // dwords i/2 + 2m and i/2 + 2m + 1
+                const uint32_t sel = (i & 1) ? 0x07060302u : 0x05040100u;
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+                    bf[i][m] = __builtin_amdgcn_perm(d[i / 2 + 2 * m + 1], d[i / 2 + 2 * m], sel);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < FW; ++i) {
+            // element offset o = i + 1
+            if constexpr (S0 == 1) {
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+                    bf[i][m] = (i & 1) ? d[(i + 1) / 2 + m]
+                                       : __builtin_amdgcn_alignbyte(d[i / 2 + 1 + m], d[i / 2 + m], 2);
+            } else {
+                const uint32_t sel = (i & 1) ? 0x05040100u : 0x07060302u;
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+                    bf[i][m] = __builtin_amdgcn_perm(d[(i + 1) / 2 + 2 * m + 1], d[(i + 1) / 2 + 2 * m], sel);
+            }
+        }
+    }
+}
+
+template <int FW, int FH, int S0, int F16, bool ALIGNED>
+__global__ __launch_bounds__(WG_THREADS, 1) void wgrad_rows(float *__restrict__ out,
+                                                            const uint16_t *__restrict__ x,
+                                                            const uint16_t *__restrict__ dy,
+                                                            WGParams p) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    uint16_t *lds_dy = reinterpret_cast<uint16_t *>(smem);
+    uint16_t *lds_x = lds_dy + WG_BK * p.dyp;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    long long bid = blockIdx.x;
+    const int split = (int)(bid % p.nsplit);
+    bid /= p.nsplit;
+    const int ct = (int)(bid % p.ctiles);
+    const int kt = (int)(bid / p.ctiles);
+
+    constexpr int RS = FW * FH;
+    wf32x4 acc[4][RS];
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+        for (int t = 0; t < RS; ++t) acc[mi][t] = wf32x4{0.f, 0.f, 0.f, 0.f};
+
+    const int g = lane >> 4, r16 = lane & 15;
+    const int cl = wave * 16 + r16;  // lane's channel within the tile
+    const int u0 = split * p.upsplit;
+    const int u1 = min(u0 + p.upsplit, p.total_units);
+    BlockRegs R;
+    if (ALIGNED && u0 < u1) {
+        const int n = u0 / p.units_per_img;
+        fetch_block(R, x, dy, p, n, (u0 - n * p.units_per_img) * p.R_oh, kt, ct);
+    }
+    for (int u = u0; u < u1; ++u) {
+        __syncthreads();  // the previous block is no longer read
+        if constexpr (ALIGNED) {
+            commit_block(R, lds_dy, lds_x, p);
+        } else {
+            const int n = u / p.units_per_img;
+            load_direct(x, dy, lds_dy, lds_x, p, n, (u - n * p.units_per_img) * p.R_oh, kt, ct);
+        }
+        __syncthreads();
+        if (ALIGNED && u + 1 < u1) {  // next block's loads overlap this block's MFMAs
+            const int n1 = (u + 1) / p.units_per_img;
+            fetch_block(R, x, dy, p, n1, (u + 1 - n1 * p.units_per_img) * p.R_oh, kt, ct);
+        }
+        for (int s = 0; s < p.steps; ++s) {
+            int seg = s * 4 + g;
+            // A fragments: dy rows mi*16 + r16, pixels seg*8 .. +7
+            wu32x4 af[4];
+#pragma unroll
+            for (int mi = 0; mi < 4; ++mi)
+                af[mi] = *reinterpret_cast<const wu32x4 *>(lds_dy + (mi * 16 + r16) * p.dyp + seg * 8);
+            if (seg >= p.nseg) seg = p.nseg - 1;  // dummy segment: A is zero there
+            const int ro = seg / p.segrow;
+            const int ow0 = (seg - ro * p.segrow) * 8;
+            const int e0 = ow0 * S0 + (p.PADL - p.p0);
+#pragma unroll
+            for (int j = 0; j < FH; ++j) {
+                const uint16_t *rowp = lds_x + (cl * p.PR + ro * p.s1 + j * p.d1) * p.PWpad;
+                wu32x4 bf[FW];
+                b_frags<FW, S0>(rowp, e0, bf);
+#pragma unroll
+                for (int i = 0; i < FW; ++i)
+#pragma unroll
+                    for (int mi = 0; mi < 4; ++mi) {
+                        if constexpr (F16)
+                            acc[mi][j * FW + i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                                __builtin_bit_cast(wf16x8, af[mi]), __builtin_bit_cast(wf16x8, bf[i]),
+                                acc[mi][j * FW + i], 0, 0, 0);
+                        else
+                            acc[mi][j * FW + i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                __builtin_bit_cast(wbf16x8, af[mi]), __builtin_bit_cast(wbf16x8, bf[i]),
+                                acc[mi][j * FW + i], 0, 0, 0);
+                    }
+            }
+        }
+    }
+
+    // acc[mi][t][v] = D[filter][channel] for tap t = j*FW + i:
+    // filter = kt*64 + mi*16 + 4*(lane/16) + v, channel = ct*64 + wave*16 + lane%16.
+    const int c = ct * WG_BC + cl;
+    if (c >= p.C) return;
+    const long long total = (long long)p.K * p.C * p.RS;
+    float *dst = out + (long long)split * total;
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int k = kt * WG_BK + mi * 16 + g * 4 + v;
+            if (k >= p.K) continue;
+            float *drow = dst + ((long long)k * p.C + c) * p.RS;
+#pragma unroll
+            for (int t = 0; t < RS; ++t) drow[RS - 1 - t] = acc[mi][t][v];
+        }
+}
+
+// Sums nsplit partial dW buffers in split order and converts to the output
+// type (deterministic: the order never depends on scheduling).
+template <typename TOut>
+__global__ __launch_bounds__(256) void wgrad_sum(TOut *__restrict__ out, const float *__restrict__ part,
+                                                 long long total, int nsplit) {
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        float s = 0.f;
+        for (int q = 0; q < nsplit; ++q) s += part[(long long)q * total + e];
+        if constexpr (sizeof(TOut) == 4)
+            out[e] = s;
+        else
+            out[e] = f32_to_bf16_bits(s);
+    }
+}
+__global__ __launch_bounds__(256) void wgrad_sum_f16(_Float16 *__restrict__ out,
+                                                     const float *__restrict__ part, long long total,
+                                                     int nsplit) {
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        float s = 0.f;
+        for (int q = 0; q < nsplit; ++q) s += part[(long long)q * total + e];
+        out[e] = (_Float16)s;
+    }
+}
+
+static size_t wg_lds(const WGParams &p) {
+    return (size_t)WG_BK * p.dyp * 2 + (size_t)WG_BC * p.PR * p.PWpad * 2;
+}
+
+// Plans the blocking; returns false when the geometry is outside this kernel.
+static bool wg_plan(WGParams &p, int fw, int s0, int d0) {
+    if (d0 != 1 || (s0 != 1 && s0 != 2) || (fw != 1 && fw != 3) || (p.fh != 1 && p.fh != 3))
+        return false;
+    p.RS = fw * p.fh;
+    if (p.OW <= 0 || p.OH <= 0 || p.N <= 0 || p.C <= 0 || p.K <= 0) return false;
+    p.OWp = (p.OW + 7) & ~7;
+    p.segrow = p.OWp / 8;
+    p.PADL = (p.p0 + 7) & ~7;
+    // widest column read: last segment start + 8 values at stride s0 + taps,
+    // plus the dword window slack
+    const int maxe = (p.OWp - 8) * s0 + (p.PADL - p.p0) + 8 * s0 + fw + 4;
+    p.PWpad = ((maxe + 7) & ~7) + 8;
+    if (p.PADL - p.p0 < 0) return false;
+    // rows per block: about 256 padded pixels, within the LDS budget
+    int r = 256 / p.OWp;
+    if (r < 1) r = 1;
+    if (r > p.OH) r = p.OH;
+    for (; r >= 1; --r) {
+        p.R_oh = r;
+        p.nseg = r * p.segrow;
+        p.steps = (p.nseg + 3) / 4;
+        p.dyp = p.steps * 32 + 8;
+        p.PR = (r - 1) * p.s1 + (p.fh - 1) * p.d1 + 1;
+        const bool regs_ok = WG_BK * p.steps * 4 <= WG_THREADS * WG_MAXD &&
+                             WG_BC * p.PR * (p.PWpad / 8) <= WG_THREADS * WG_MAXX;
+        if (wg_lds(p) <= WG_LDS_LIMIT && regs_ok) break;
+    }
+    if (r < 1) return false;
+    p.units_per_img = (p.OH + p.R_oh - 1) / p.R_oh;
+    p.total_units = p.units_per_img * p.N;
+    p.ktiles = (p.K + WG_BK - 1) / WG_BK;
+    p.ctiles = (p.C + WG_BC - 1) / WG_BC;
+    const int tiles = p.ktiles * p.ctiles;
+    int ns = (256 + tiles - 1) / tiles;
+    if (ns > p.total_units) ns = p.total_units;
+    if (ns < 1) ns = 1;
+    p.upsplit = (p.total_units + ns - 1) / ns;
+    p.nsplit = (p.total_units + p.upsplit - 1) / p.upsplit;
+    return true;
+}
+
+static void wg_fill(WGParams &p, const long long *xd, const long long *wd, const long long *yd,
+                    const long long *st, const long long *pd, const long long *dl) {
+    p.W = (int)xd[0];
+    p.H = (int)xd[1];
+    p.C = (int)xd[2];
+    p.N = (int)xd[3];
+    p.OW = (int)yd[0];
+    p.OH = (int)yd[1];
+    p.K = (int)wd[3];
+    p.fh = (int)wd[1];
+    p.s1 = (int)st[1];
+    p.p0 = (int)pd[0];
+    p.p1 = (int)pd[1];
+    p.d1 = (int)dl[1];
+}
+
+template <int FW, int FH, int S0, int F16>
+static int wg_launch(float *part, const void *x, const void *dy, const WGParams &p, hipStream_t s) {
+    const bool aligned = (p.W & 7) == 0 && (p.OW & 7) == 0;
+    auto fn = aligned ? wgrad_rows<FW, FH, S0, F16, true> : wgrad_rows<FW, FH, S0, F16, false>;
+    static bool attr[2] = {false, false};
+    if (!attr[aligned]) {
+        if (hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)WG_LDS_LIMIT) != hipSuccess)
+            return K_ERR_RUNTIME;
+        attr[aligned] = true;
+    }
+    const long long grid = (long long)p.ktiles * p.ctiles * p.nsplit;
+    fn<<<dim3((unsigned)grid), WG_THREADS, wg_lds(p), s>>>(part, (const uint16_t *)x,
+                                                          (const uint16_t *)dy, p);
+    return launch_status();
+}
+
+}  // namespace afhip
+
+using namespace afhip;
+
+extern "C" long long afhip_k_conv2_nn_wgrad_rows_workspace(int in_type, int precision,
+                                                          const long long *xd, const long long *wd,
+                                                          const long long *yd, const long long *st,
+                                                          const long long *pd, const long long *dl) {
+    const bool bf = (in_type == DT_BF16 && (precision == 0 || precision == 1));
+    const bool hf = (in_type == DT_F16 && (precision == 0 || precision == 2));
+    if (!bf && !hf) return 0;
+    WGParams p;
+    wg_fill(p, xd, wd, yd, st, pd, dl);
+    if (!wg_plan(p, (int)wd[0], (int)st[0], (int)dl[0])) return 0;
+    return (long long)p.nsplit * p.K * p.C * p.RS * 4;
+}
+
+extern "C" int afhip_k_conv2_nn_wgrad_rows(int in_type, int out_type, int precision, void *dw,
+                                           const void *x, const void *dy, const long long *xd,
+                                           const long long *wd, const long long *yd,
+                                           const long long *st, const long long *pd,
+                                           const long long *dl, void *workspace,
+                                           long long workspace_bytes, void *stream) {
+    const long long need = afhip_k_conv2_nn_wgrad_rows_workspace(in_type, precision, xd, wd, yd, st, pd, dl);
+    if (need == 0) return K_ERR_NOT_SUPPORTED;
+    if (workspace == nullptr || workspace_bytes < need) return K_ERR_ARG;
+    WGParams p;
+    wg_fill(p, xd, wd, yd, st, pd, dl);
+    const int fw = (int)wd[0], s0 = (int)st[0];
+    wg_plan(p, fw, s0, (int)dl[0]);
+    p.is_f16 = in_type == DT_F16;
+    hipStream_t s = (hipStream_t)stream;
+    float *part = (float *)workspace;
+    int rc;
+#define WGO(FWV, FHV, S0V)                                                                    \
+    rc = p.is_f16 ? wg_launch<FWV, FHV, S0V, 1>(part, x, dy, p, s)                           \
+                  : wg_launch<FWV, FHV, S0V, 0>(part, x, dy, p, s)
+#define WGS(FWV, FHV)            \
+    if (s0 == 1) WGO(FWV, FHV, 1); \
+    else WGO(FWV, FHV, 2);
+    if (fw == 3 && p.fh == 3) { WGS(3, 3) }
+    else if (fw == 3) { WGS(3, 1) }
+    else if (p.fh == 3) { WGS(1, 3) }
+    else { WGS(1, 1) }
+#undef WGS
+#undef WGO
+    if (rc) return rc;
+    const long long total = (long long)p.K * p.C * p.RS;
+    long long blocks = (total + 255) / 256;
+    if (blocks > kMaxGridBlocks) blocks = kMaxGridBlocks;
+    if (out_type == DT_F32)
+        wgrad_sum<float><<<(unsigned)blocks, 256, 0, s>>>((float *)dw, part, total, p.nsplit);
+    else if (out_type == DT_BF16 && in_type == DT_BF16)
+        wgrad_sum<uint16_t><<<(unsigned)blocks, 256, 0, s>>>((uint16_t *)dw, part, total, p.nsplit);
+    else if (out_type == DT_F16 && in_type == DT_F16)
+        wgrad_sum_f16<<<(unsigned)blocks, 256, 0, s>>>((_Float16 *)dw, part, total, p.nsplit);
+    else
+        return K_ERR_TYPE;
+    return launch_status();
+}

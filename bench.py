@@ -1,0 +1,304 @@
+"""Headline benchmark: convolve2_nn forward + backward on one MI355X.
+
+One step = af_convolve2_nn forward, then af_convolve2_gradient_nn for the data
+gradient and for the filter gradient, on inputs already resident in HBM, for
+N=32 images of 64 channels at 56x56 with 128 filters of 3x3, stride 1, pad 1
+(the reference's convolve2_nn configuration). The timed arithmetic is BF16
+operands on the matrix cores with FP32 accumulation; the same step is also
+timed in FP32 (the plain path) for the BF16/FP32 ratio.
+
+value = algorithmic FLOPs of (fwd + dgrad + wgrad) / step time, in TFLOP/s,
+where each pass counts 2*N*K*OH*OW*C*fh*fw FLOPs.
+
+Prints ONE JSON line. --gpus N with N > 1 exits: this build runs one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Workload: x [W,H,C,N], w [fw,fh,C,K]
+XD = (56, 56, 64, 32)
+WD = (3, 3, 64, 128)
+ST, PD, DL = (1, 1), (1, 1), (1, 1)
+SEED = 20261202 + 2
+PEAK = {"hbm_GBs": 8000.0, "bf16_TFs": 2500.0, "fp32_TFs": 157.3}
+
+
+def nn_flops(xd, wd, yd):
+    return 2.0 * xd[3] * wd[3] * yd[0] * yd[1] * xd[2] * wd[0] * wd[1]
+
+
+class HipEvents:
+    """hipEvent pairs on an explicit stream (the library's stream)."""
+
+    def __init__(self):
+        self.h = ctypes.CDLL("libamdhip64.so")
+        self.h.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.h.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.h.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.h.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
+                                               ctypes.c_void_p]
+        self.h.hipEventDestroy.argtypes = [ctypes.c_void_p]
+
+    def create(self, n):
+        evs = []
+        for _ in range(n):
+            e = ctypes.c_void_p()
+            assert self.h.hipEventCreate(ctypes.byref(e)) == 0
+            evs.append(e)
+        return evs
+
+    def record(self, e, stream):
+        assert self.h.hipEventRecord(e, ctypes.c_void_p(stream)) == 0
+
+    def ms(self, a, b):
+        assert self.h.hipEventSynchronize(b) == 0
+        t = ctypes.c_float()
+        assert self.h.hipEventElapsedTime(ctypes.byref(t), a, b) == 0
+        return float(t.value)
+
+
+def run_step(af, x, w, dy, y_holder, precision, out_type):
+    y = af.convolve2NN(x, w, ST, PD, DL, precision=precision, out_type=out_type)
+    dx = af.convolve2GradientNN(dy, x, w, y, ST, PD, DL, af.AF_CONV_GRADIENT_DATA,
+                                precision=precision, out_type=out_type)
+    dw = af.convolve2GradientNN(dy, x, w, y, ST, PD, DL, af.AF_CONV_GRADIENT_FILTER,
+                                precision=precision, out_type=out_type)
+    y_holder[:] = [y, dx, dw]
+
+
+def time_path(af, ev, stream, x, w, dy, precision, out_type, steps, warmup):
+    hold = []
+    for _ in range(warmup):
+        run_step(af, x, w, dy, hold, precision, out_type)
+    af.sync()
+    # per-pass events: e[3*s + p] .. e[3*s + p + 1]
+    marks = ev.create(3 * steps + 1)
+    t0 = time.perf_counter()
+    ev.record(marks[0], stream)
+    for s in range(steps):
+        y = af.convolve2NN(x, w, ST, PD, DL, precision=precision, out_type=out_type)
+        ev.record(marks[3 * s + 1], stream)
+        dx = af.convolve2GradientNN(dy, x, w, y, ST, PD, DL, af.AF_CONV_GRADIENT_DATA,
+                                    precision=precision, out_type=out_type)
+        ev.record(marks[3 * s + 2], stream)
+        dw = af.convolve2GradientNN(dy, x, w, y, ST, PD, DL, af.AF_CONV_GRADIENT_FILTER,
+                                    precision=precision, out_type=out_type)
+        ev.record(marks[3 * s + 3], stream)
+        hold = [y, dx, dw]
+    af.sync()
+    wall = time.perf_counter() - t0
+    per = np.zeros(3)
+    for s in range(steps):
+        for p in range(3):
+            per[p] += ev.ms(marks[3 * s + p], marks[3 * s + p + 1])
+    total_ms = ev.ms(marks[0], marks[-1])
+    return total_ms / steps, per / steps, wall, hold
+
+
+def fp64_errors(x, w, dy, outs):
+    """rel-L2 of (y, dx, dw) against FP64 evaluations of the closed forms
+    (stride 1, pad 1, dilation 1), written with tensordot for speed."""
+    import oracle
+
+    y, dx, dw = [_as_f64(o) for o in outs]
+    y64 = _fwd64(x, w)
+    dx64 = _dgrad64(dy, w, x.shape)
+    dw64 = _wgrad64(x, dy, w.shape)
+    return (oracle.rel_l2(y, y64), oracle.rel_l2(dx, dx64), oracle.rel_l2(dw, dw64))
+
+
+def _as_f64(a):
+    v = a.to_numpy()
+    if v.dtype == np.uint16:  # bfloat16 bits
+        v = (v.astype(np.uint32) << 16).view(np.float32)
+    return v.astype(np.float64)
+
+
+def _fwd64(x, w):
+    x = x.astype(np.float64)
+    w = w.astype(np.float64)
+    W, H, C, N = x.shape
+    fw, fh, _, K = w.shape
+    xp = np.zeros((W + 2, H + 2, C, N))
+    xp[1:-1, 1:-1] = x
+    y = np.zeros((W, H, K, N))
+    for i in range(fw):
+        for j in range(fh):
+            xs = xp[i:i + W, j:j + H]                       # [W,H,C,N]
+            y += np.tensordot(xs, w[fw - 1 - i, fh - 1 - j], axes=([2], [0])).transpose(0, 1, 3, 2)
+    return y
+
+
+def _dgrad64(dy, w, xshape):
+    dy = dy.astype(np.float64)
+    w = w.astype(np.float64)
+    W, H, C, N = xshape
+    fw, fh, _, K = w.shape
+    dxp = np.zeros((W + 2, H + 2, C, N))
+    for i in range(fw):
+        for j in range(fh):
+            t = np.tensordot(dy, w[fw - 1 - i, fh - 1 - j], axes=([2], [1]))  # [W,H,N,C]
+            dxp[i:i + W, j:j + H] += t.transpose(0, 1, 3, 2)
+    return dxp[1:-1, 1:-1]
+
+
+def _wgrad64(x, dy, wshape):
+    x = x.astype(np.float64)
+    dy = dy.astype(np.float64)
+    W, H, C, N = x.shape
+    fw, fh, _, K = wshape
+    xp = np.zeros((W + 2, H + 2, C, N))
+    xp[1:-1, 1:-1] = x
+    dw = np.zeros(wshape)
+    for i in range(fw):
+        for j in range(fh):
+            xs = xp[i:i + W, j:j + H]
+            dw[fw - 1 - i, fh - 1 - j] = np.tensordot(xs, dy, axes=([0, 1, 3], [0, 1, 3]))
+    return dw
+
+
+def cpu_baseline(budget_s=15.0):
+    """The oracle's restatement of the reference CPU backend (im2col + numpy
+    SGEMM + col2im) on a bounded sample: the same layer with fewer images."""
+    import oracle
+
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max([i.get("num_threads", 1) for i in threadpool_info()
+                       if i.get("user_api") == "blas"] or [1])
+    except Exception:  # pragma: no cover
+        threads = os.cpu_count() or 1
+    rng = np.random.default_rng(SEED)
+    n = 4
+    xd = XD[:3] + (n,)
+    x = rng.standard_normal(xd).astype(np.float32)
+    w = (rng.standard_normal(WD) * np.sqrt(2.0 / (WD[2] * 9))).astype(np.float32)
+    y = oracle.conv2_nn(x, w, ST, PD, DL)
+    dy = rng.standard_normal(y.shape).astype(np.float32)
+    flops = 3 * nn_flops(xd, WD, y.shape)
+    reps, t = 0, 0.0
+    t0 = time.perf_counter()
+    while t < budget_s and reps < 200:
+        oracle.conv2_nn(x, w, ST, PD, DL)
+        oracle.conv2_data_grad(dy, x, w, ST, PD, DL)
+        oracle.conv2_filter_grad(dy, x, w, ST, PD, DL)
+        reps += 1
+        t = time.perf_counter() - t0
+    return {"value": flops * reps / t / 1e12, "unit": "TFLOP/s", "cores": int(threads),
+            "kind": "port",
+            "sample": f"FP32 fwd+dgrad+wgrad on N={n} images of the same layer "
+                      f"(x {list(xd)}, w {list(WD)}), {reps} reps in {t:.1f} s; "
+                      "im2col/col2im single-threaded C, SGEMM on numpy's OpenBLAS"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-check", action="store_true", help="skip the FP64 error check")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU sharding is a design note, not a runnable path.", file=sys.stderr)
+        sys.exit(2)
+
+    import arrayfire_amd as af
+
+    af.lib()
+    stream = af.stream()
+    ev = HipEvents()
+    rng = np.random.default_rng(SEED)
+    x = rng.standard_normal(XD).astype(np.float32)
+    w = (rng.standard_normal(WD) * np.sqrt(2.0 / (WD[2] * WD[0] * WD[1]))).astype(np.float32)
+    yd = (XD[0], XD[1], WD[3], XD[3])
+    dy = rng.standard_normal(yd).astype(np.float32)
+    X32, W32, DY32 = af.Array(x), af.Array(w), af.Array(dy)
+    Xb, Wb, DYb = X32.astype(af.bf16), W32.astype(af.bf16), DY32.astype(af.bf16)
+    af.sync()
+
+    flops_pass = nn_flops(XD, WD, yd)
+    # BF16 step: bf16 operands and bf16 results (the af convention: the output
+    # dtype is the input dtype), FP32 accumulation inside every kernel
+    ms_bf16, per_bf16, wall_bf16, outs = time_path(
+        af, ev, stream, Xb, Wb, DYb, af.PRECISION_DEFAULT, af.bf16, args.steps, args.warmup)
+    ms_f32, per_f32, _, outs32 = time_path(
+        af, ev, stream, X32, W32, DY32, af.PRECISION_DEFAULT, af.f32, max(3, args.steps // 2),
+        max(1, args.warmup // 2))
+    tf_bf16 = 3 * flops_pass / (ms_bf16 * 1e-3) / 1e12
+    tf_f32 = 3 * flops_pass / (ms_f32 * 1e-3) / 1e12
+
+    errs = None
+    if not args.no_check:
+        # the BF16 operands are the RNE roundings of x, w, dy: the error
+        # against FP64 includes that input rounding
+        errs = fp64_errors(x, w, dy, outs)
+
+    # roofline of the dominant pass (largest average time)
+    names = ["fwd", "dgrad", "wgrad"]
+    k = int(np.argmax(per_bf16))
+    # algorithmic bytes: bf16 operands read once, bf16 result written once
+    nx, nw, ny = np.prod(XD), np.prod(WD), np.prod(yd)
+    bytes_pass = {"fwd": 2 * (nx + nw + ny),
+                  "dgrad": 2 * (ny + nw + nx),
+                  "wgrad": 2 * (nx + ny + nw)}[names[k]]
+    t_s = per_bf16[k] * 1e-3
+    t_mem = bytes_pass / (PEAK["hbm_GBs"] * 1e9)
+    t_cmp = flops_pass / (PEAK["bf16_TFs"] * 1e12)
+    if t_mem >= t_cmp:
+        roof = {"bound": "hbm", "achieved": bytes_pass / t_s / 1e9, "peak": PEAK["hbm_GBs"],
+                "unit": "GB/s"}
+    else:
+        roof = {"bound": "mfma", "achieved": flops_pass / t_s / 1e12, "peak": PEAK["bf16_TFs"],
+                "unit": "TFLOP/s"}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = None
+    roof["kernel"] = f"convolve2_nn {names[k]} (BF16 MFMA implicit GEMM)"
+    roof["algorithmic_bytes"] = int(bytes_pass)
+    roof["algorithmic_flops"] = flops_pass
+
+    out = {
+        "metric": "convolve2_nn fwd+bwd TFLOP/s, BF16-MFMA vs FP32 plain, at rel-L2 <=1e-2",
+        "value": round(tf_bf16, 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_bf16, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic (seeded normal inputs, He-scaled filters)",
+        "config": {"workload": "convolve2_nn fwd+dgrad+wgrad, x [56,56,64,32] (N=32, C=64, "
+                               "56x56), w [3,3,64,128], stride 1, pad 1, dilation 1; bf16 "
+                               "operands and results, fp32 accumulation",
+                   "global_batch": XD[3], "parallelism": "none (1 GPU)"},
+        "fp32_plain_tflops": round(tf_f32, 3),
+        "bf16_over_fp32": round(tf_bf16 / tf_f32, 3),
+        "ms_per_pass_bf16": {n: round(float(v), 4) for n, v in zip(names, per_bf16)},
+        "ms_per_pass_fp32": {n: round(float(v), 4) for n, v in zip(names, per_f32)},
+        "host_wall_ms_per_step_bf16": round(wall_bf16 / args.steps * 1e3, 4),
+        "rel_l2_vs_fp64_bf16": None if errs is None else {n: float(f"{e:.3e}") for n, e in
+                                                         zip(names, errs)},
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -47,8 +47,8 @@ AFAPI af_err af_cast(af_array *out, const af_array in, const af_dtype type);
 /* af_convolve2_nn with an explicit arithmetic. signal and filter must have
  * the same dtype (f32, f16 or afhip_bf16; f64 only with
  * AFHIP_PRECISION_DEFAULT). out_type selects the output dtype (f32, or the
- * input dtype when it is f16/afhip_bf16); pass (af_dtype)-1 for the default,
- * which is the input dtype for f32/f16/f64 and f32 for afhip_bf16 inputs. */
+ * input dtype when it is f16/afhip_bf16); (af_dtype)-1 means the input
+ * dtype, as af_convolve2_nn does. */
 AFAPI af_err afhip_convolve2_nn_ex(af_array *out, const af_array signal,
                                    const af_array filter,
                                    const unsigned stride_dims,
@@ -60,8 +60,9 @@ AFAPI af_err afhip_convolve2_nn_ex(af_array *out, const af_array signal,
                                    afhip_precision precision, af_dtype out_type);
 
 /* af_convolve2_gradient_nn with an explicit arithmetic (see above). The
- * gradient is returned in out_type ((af_dtype)-1: as for
- * afhip_convolve2_nn_ex). */
+ * incoming gradient, signal and filter share one dtype; convolved_output is
+ * read for its dims only (it may be the FP32 output of a BF16 forward). The
+ * gradient is returned in out_type ((af_dtype)-1: the input dtype). */
 AFAPI af_err afhip_convolve2_gradient_nn_ex(
     af_array *out, const af_array incoming_gradient,
     const af_array original_signal, const af_array original_filter,

@@ -123,3 +123,47 @@ AFHIP_KAPI int afhip_k_conv2_nn_f64(int pass, void *out, const void *a0,
                                     const long long *stride,
                                     const long long *pad, const long long *dil,
                                     void *stream);
+
+/* Patch-staged implicit GEMM for the forward pass (pass 0) and the stride-1
+ * data gradient (pass 1), the fast path behind afhip_k_conv2_nn_fwd/_dgrad.
+ * The workspace query returns the bytes of packed weights the call needs, or
+ * 0 when the geometry is not served (stride != 1 for pass 1, or an input
+ * patch that does not fit in LDS); the call then returns 301 and the caller
+ * uses afhip_k_conv2_nn_fwd/_dgrad. For pass 0 `in` is x and `out` is y; for
+ * pass 1 `in` is dy and `out` is dx. */
+AFHIP_KAPI long long afhip_k_conv2_nn_patch_workspace(
+    int pass, int in_type, int precision, const long long *xdims,
+    const long long *wdims, const long long *ydims, const long long *stride,
+    const long long *pad, const long long *dil);
+AFHIP_KAPI int afhip_k_conv2_nn_patch(int pass, int in_type, int out_type,
+                                      int precision, void *out, const void *in,
+                                      const void *w, const long long *xdims,
+                                      const long long *wdims,
+                                      const long long *ydims,
+                                      const long long *stride,
+                                      const long long *pad,
+                                      const long long *dil, void *workspace,
+                                      long long workspace_bytes, void *stream);
+
+/* Row-staged filter gradient for 16-bit operands (bfloat16 or binary16
+ * storage, FP32 accumulation): the fast path behind afhip_k_conv2_nn_wgrad
+ * for filters of width 1 or 3, dilation 1 along W, stride 1 or 2 along W and
+ * at most 9 taps. The workspace holds the FP32 partial tiles of the split
+ * reduction; the query returns 0 when the geometry or dtype is not served
+ * (the call then returns 301). */
+AFHIP_KAPI long long afhip_k_conv2_nn_wgrad_rows_workspace(
+    int in_type, int precision, const long long *xdims, const long long *wdims,
+    const long long *ydims, const long long *stride, const long long *pad,
+    const long long *dil);
+AFHIP_KAPI int afhip_k_conv2_nn_wgrad_rows(int in_type, int out_type,
+                                           int precision, void *dw,
+                                           const void *x, const void *dy,
+                                           const long long *xdims,
+                                           const long long *wdims,
+                                           const long long *ydims,
+                                           const long long *stride,
+                                           const long long *pad,
+                                           const long long *dil,
+                                           void *workspace,
+                                           long long workspace_bytes,
+                                           void *stream);

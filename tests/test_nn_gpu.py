@@ -272,3 +272,56 @@ def test_separable_arg_errors(af):
         af.convolve_sep(np.ones((3, 3), np.float32), np.ones(3, np.float32),
                         np.ones((8, 8), np.float32))
     assert e.value.code == af.AF_ERR_ARG
+
+
+@pytest.mark.parametrize("prec", [None, 1, 2, 3], ids=["f32", "bf16", "f16", "split"])
+@pytest.mark.parametrize("g", GEOMS, ids=_id)
+def test_patch_path_matches_gather_path(af, rng, g, prec, monkeypatch):
+    """The LDS-patch kernels (forward, stride-1 data gradient) and the gather
+    kernels compute the same sums in different orders."""
+    x, w = _data(rng, g)
+    y = af.convolve2NN(x, w, g[2], g[3], g[4], precision=prec)
+    dy = rng.standard_normal(y.dims).astype(np.float32)
+    dx = af.convolve2GradientNN(dy, x, w, y, *g[2:], af.AF_CONV_GRADIENT_DATA, precision=prec)
+    monkeypatch.setenv("AFHIP_NN_GATHER", "1")
+    y2 = af.convolve2NN(x, w, g[2], g[3], g[4], precision=prec)
+    dx2 = af.convolve2GradientNN(dy, x, w, y, *g[2:], af.AF_CONV_GRADIENT_DATA, precision=prec)
+    assert oracle.rel_l2(y.to_numpy(), y2.to_numpy()) <= 2e-6
+    assert oracle.rel_l2(dx.to_numpy(), dx2.to_numpy()) <= 2e-6
+
+
+WGRAD_GEOMS = GEOMS + [
+    ((56, 56, 64, 3), (3, 3, 64, 128), (1, 1), (1, 1), (1, 1)),   # headline layer, N=3
+    ((28, 28, 128, 2), (3, 3, 128, 128), (2, 2), (1, 1), (1, 1)),  # stride-2 layer, N=2
+    ((30, 9, 70, 2), (3, 3, 70, 66), (2, 1), (0, 2), (1, 2)),
+    ((13, 11, 5, 2), (1, 3, 5, 9), (2, 2), (0, 1), (1, 1)),
+]
+
+
+@pytest.mark.parametrize("dt", ["bf16", "f16"])
+@pytest.mark.parametrize("g", WGRAD_GEOMS, ids=_id)
+def test_filter_gradient_16bit_arrays(af, rng, g, dt, monkeypatch):
+    """16-bit arrays take the row-staged filter-gradient kernel where it
+    applies: against the FP64 closed form of the same rounded operands (the
+    only error left is FP32 accumulation and the final rounding), and against
+    the gather kernel."""
+    x, w = _data(rng, g)
+    t = af.bf16 if dt == "bf16" else af.f16
+    X, Wt = af.Array(x).astype(t), af.Array(w).astype(t)
+    y = af.convolve2NN(X, Wt, g[2], g[3], g[4], out_type=af.f32)
+    dy = rng.standard_normal(y.dims).astype(np.float32)
+    DY = af.Array(dy).astype(t)
+    dw = af.convolve2GradientNN(DY, X, Wt, y, *g[2:], af.AF_CONV_GRADIENT_FILTER, out_type=af.f32)
+    xr, dyr = _rounded(X), _rounded(DY)
+    ref = oracle.conv2_filter_grad_direct_f64(dyr, xr, w, *g[2:])
+    assert oracle.rel_l2(dw.to_numpy(), ref) <= 1e-5
+    monkeypatch.setenv("AFHIP_NN_GATHER", "1")
+    dw2 = af.convolve2GradientNN(DY, X, Wt, y, *g[2:], af.AF_CONV_GRADIENT_FILTER, out_type=af.f32)
+    assert oracle.rel_l2(dw.to_numpy(), dw2.to_numpy()) <= 2e-6
+
+
+def _rounded(a):
+    v = a.to_numpy()
+    if v.dtype == np.uint16:
+        return (v.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+    return v.astype(np.float64)
