@@ -143,6 +143,14 @@ def lib() -> ctypes.CDLL:
         L.afhip_k_conv2_nn_wgrad_workspace.restype = ll
         L.afhip_k_conv2_nn_wgrad.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, vp,
                                              i64p, i64p, i64p, i64p, i64p, i64p, vp, ll, vp]
+        L.afhip_k_conv2_nn_patch_workspace.argtypes = [ctypes.c_int] * 3 + [i64p] * 6
+        L.afhip_k_conv2_nn_patch_workspace.restype = ll
+        L.afhip_k_conv2_nn_patch.argtypes = [ctypes.c_int] * 4 + [vp, vp, vp] + [i64p] * 6 + [vp, ll, vp]
+        L.afhip_k_conv2_nn_wgrad_rows_workspace.argtypes = [ctypes.c_int] * 2 + [i64p] * 6
+        L.afhip_k_conv2_nn_wgrad_rows_workspace.restype = ll
+        L.afhip_k_conv2_nn_wgrad_rows.argtypes = [ctypes.c_int] * 3 + [vp, vp, vp] + [i64p] * 6 + [vp, ll, vp]
+        L.afhip_k_conv2_nn_f64.argtypes = [ctypes.c_int, vp, vp, vp] + [i64p] * 6 + [vp]
+        L.afhip_k_scale.argtypes = [ctypes.c_int, vp, vp, ctypes.c_double, ll, vp]
         _lib = L
     return _lib
 

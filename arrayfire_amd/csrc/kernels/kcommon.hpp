@@ -75,4 +75,56 @@ __device__ inline long long xcd_remap(long long bid, long long nblk) {
     return xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
 }
 
+// Loads an LH x LW halo tile of an FP32 image into LDS (row pitch LS) with
+// zeros outside [0,S0) x [0,S1). The tile origin (ix0, iy0) may be negative
+// or unaligned: rows are read as aligned float4s (when S0 % 4 == 0) that
+// cover the tile's columns, eight float4 slots per lane in flight before any
+// LDS store, then scattered into place. NQ (float4 slots per row) must be at
+// least (LW + 6) / 4; a compile-time NQ keeps the slot decomposition cheap.
+template <int NT, int NQ>
+__device__ inline void load_halo_tile_f32(float *tile, const float *__restrict__ img, long long S0,
+                                          long long S1, long long ix0, long long iy0, int LH, int LW,
+                                          int LS) {
+    const long long a0 = ix0 - (((ix0 % 4) + 4) % 4);  // floor to a multiple of 4
+    const int off = (int)(ix0 - a0);                  // 0..3
+    const int slots = LH * NQ;
+    const bool vec = (S0 & 3) == 0;
+    for (int s0 = 0; s0 < slots; s0 += 8 * NT) {
+        float4 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int slot = s0 + u * NT + (int)threadIdx.x;
+            v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+            const int r = slot / NQ, q = slot - (slot / NQ) * NQ;
+            if (slot < slots && 4 * q - off < LW) {
+                const long long gy = iy0 + r;
+                const long long gx = a0 + 4 * q;
+                if (gy >= 0 && gy < S1) {
+                    const float *row = img + gy * S0;
+                    if (vec && gx >= 0 && gx + 3 < S0) {
+                        v[u] = *reinterpret_cast<const float4 *>(row + gx);
+                    } else {
+                        float t[4];
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) t[e] = (gx + e >= 0 && gx + e < S0) ? row[gx + e] : 0.f;
+                        v[u] = make_float4(t[0], t[1], t[2], t[3]);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int slot = s0 + u * NT + (int)threadIdx.x;
+            const int r = slot / NQ, q = slot - (slot / NQ) * NQ;
+            const int c0 = 4 * q - off;
+            if (slot < slots && c0 < LW) {
+                const float t[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (c0 + e >= 0 && c0 + e < LW) tile[r * LS + c0 + e] = t[e];
+            }
+        }
+    }
+}
+
 }  // namespace afhip

@@ -43,124 +43,75 @@ struct WGParams {
     int is_f16;
 };
 
-constexpr int WG_MAXD = 8;   // dy chunks (8 values) per thread per block
-constexpr int WG_MAXX = 16;  // x chunks per thread per block
-
-// Register staging of one block: fetch() issues every global load of the
-// block (independent, so their latencies overlap each other and the MFMA work
-// of the block being computed), commit() stores them to LDS.
-struct BlockRegs {
-    uint4 d[WG_MAXD];
-    uint4 x[WG_MAXX];
-};
-
 __device__ inline uint4 pack8(const uint16_t *h) {
     return make_uint4(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16),
                       h[4] | ((uint32_t)h[5] << 16), h[6] | ((uint32_t)h[7] << 16));
 }
 
-// Aligned rows only (W and OW multiples of 8): every 8-value chunk of a row is
-// a single aligned 16-byte load or lies wholly outside the row.
-__device__ inline void fetch_block(BlockRegs &R, const uint16_t *__restrict__ x,
-                                   const uint16_t *__restrict__ dy, const WGParams &p, int n, int oh0,
-                                   int kt, int ct) {
-    const int tid = threadIdx.x;
-    const long long HW = (long long)p.H * p.W, OHW = (long long)p.OH * p.OW;
-    const int segs = p.steps * 4;
+// Stages one block (dy rows and the x rows they touch) in LDS. With ALIGNED
+// (W and OW multiples of 8) every 8-value chunk is one aligned 16-byte load or
+// lies wholly outside its row; otherwise chunks are gathered element-wise.
+// NB chunks per thread are loaded before any is stored so that their
+// latencies overlap; the other workgroup on the CU computes meanwhile.
+template <bool ALIGNED>
+__device__ inline uint4 dy_chunk(const uint16_t *__restrict__ dy, const WGParams &p, int e, int n,
+                                 int oh0, int kt, int segs, long long OHW) {
+    const int m = e / segs, seg = e - (e / segs) * segs;
+    const int k = kt * WG_BK + m;
+    if (m >= WG_BK || seg >= p.nseg || k >= p.K) return make_uint4(0, 0, 0, 0);
+    const int oh = oh0 + seg / p.segrow;
+    const int ow0 = (seg % p.segrow) * 8;
+    if (oh >= p.OH) return make_uint4(0, 0, 0, 0);
+    const uint16_t *src = dy + ((long long)n * p.K + k) * OHW + (long long)oh * p.OW + ow0;
+    if constexpr (ALIGNED) {
+        return *reinterpret_cast<const uint4 *>(src);
+    } else {
+        uint16_t h[8];
 #pragma unroll
-    for (int q = 0; q < WG_MAXD; ++q) {
-        const int e = tid + q * WG_THREADS;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        const int m = e / segs;
-        const int seg = e - m * segs;
-        const int k = kt * WG_BK + m;
-        if (m < WG_BK && seg < p.nseg && k < p.K) {
-            const int oh = oh0 + seg / p.segrow;
-            const int ow0 = (seg % p.segrow) * 8;
-            if (oh < p.OH) {
-                const uint16_t *src = dy + ((long long)n * p.K + k) * OHW + (long long)oh * p.OW + ow0;
-                v = *reinterpret_cast<const uint4 *>(src);
-            }
-        }
-        R.d[q] = v;
-    }
-    const int qrow = p.PWpad / 8;
-    const int xchunks = WG_BC * p.PR * qrow;
-#pragma unroll
-    for (int q = 0; q < WG_MAXX; ++q) {
-        const int e = tid + q * WG_THREADS;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (e < xchunks) {
-            const int cr = e / qrow;
-            const int qq = e - cr * qrow;
-            const int c = cr / p.PR, r = cr - (cr / p.PR) * p.PR;
-            const int ch = ct * WG_BC + c;
-            const int ih = oh0 * p.s1 - p.p1 + r;
-            const int iw0 = qq * 8 - p.PADL;
-            if (ch < p.C && ih >= 0 && ih < p.H && iw0 + 8 > 0 && iw0 < p.W) {
-                const uint16_t *src = x + ((long long)n * p.C + ch) * HW + (long long)ih * p.W;
-                v = *reinterpret_cast<const uint4 *>(src + iw0);
-            }
-        }
-        R.x[q] = v;
+        for (int t = 0; t < 8; ++t) h[t] = (ow0 + t < p.OW) ? src[t] : (uint16_t)0;
+        return pack8(h);
     }
 }
 
-__device__ inline void commit_block(const BlockRegs &R, uint16_t *lds_dy, uint16_t *lds_x,
-                                    const WGParams &p) {
-    const int tid = threadIdx.x;
-    const int segs = p.steps * 4;
+template <bool ALIGNED>
+__device__ inline uint4 x_chunk(const uint16_t *__restrict__ x, const WGParams &p, int e, int n,
+                                int oh0, int ct, int qrow, int xchunks, long long HW) {
+    if (e >= xchunks) return make_uint4(0, 0, 0, 0);
+    const int cr = e / qrow, qq = e - (e / qrow) * qrow;
+    const int c = cr / p.PR, r = cr - (cr / p.PR) * p.PR;
+    const int ch = ct * WG_BC + c;
+    const int ih = oh0 * p.s1 - p.p1 + r;
+    const int iw0 = qq * 8 - p.PADL;
+    if (ch >= p.C || ih < 0 || ih >= p.H || iw0 + 8 <= 0 || iw0 >= p.W) return make_uint4(0, 0, 0, 0);
+    const uint16_t *src = x + ((long long)n * p.C + ch) * HW + (long long)ih * p.W;
+    if constexpr (ALIGNED) {
+        return *reinterpret_cast<const uint4 *>(src + iw0);
+    } else {
+        uint16_t h[8];
 #pragma unroll
-    for (int q = 0; q < WG_MAXD; ++q) {
-        const int e = tid + q * WG_THREADS;
-        const int m = e / segs;
-        const int seg = e - m * segs;
-        if (m < WG_BK) *reinterpret_cast<uint4 *>(lds_dy + m * p.dyp + seg * 8) = R.d[q];
-    }
-    const int qrow = p.PWpad / 8;
-    const int xchunks = WG_BC * p.PR * qrow;
-#pragma unroll
-    for (int q = 0; q < WG_MAXX; ++q) {
-        const int e = tid + q * WG_THREADS;
-        if (e < xchunks) {
-            const int cr = e / qrow;
-            const int qq = e - cr * qrow;
-            *reinterpret_cast<uint4 *>(lds_x + cr * p.PWpad + qq * 8) = R.x[q];
+        for (int t = 0; t < 8; ++t) {
+            const int iw = iw0 + t;
+            h[t] = (iw >= 0 && iw < p.W) ? src[iw] : (uint16_t)0;
         }
+        return pack8(h);
     }
 }
 
-// Unaligned rows (W or OW not a multiple of 8): element loads straight to
-// LDS, four chunks in flight at a time. Not overlapped with the MFMA work.
+template <bool ALIGNED>
 __device__ inline void load_direct(const uint16_t *__restrict__ x, const uint16_t *__restrict__ dy,
                                    uint16_t *lds_dy, uint16_t *lds_x, const WGParams &p, int n, int oh0,
                                    int kt, int ct) {
+    constexpr int NB = ALIGNED ? 8 : 4;
     const int tid = threadIdx.x;
     const long long HW = (long long)p.H * p.W, OHW = (long long)p.OH * p.OW;
     const int segs = p.steps * 4;
     const int dchunks = WG_BK * segs;
-    for (int e0 = 0; e0 < dchunks; e0 += 4 * WG_THREADS) {
-        uint4 v[4];
+    for (int e0 = 0; e0 < dchunks; e0 += NB * WG_THREADS) {
+        uint4 v[NB];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int e = e0 + q * WG_THREADS + tid;
-            const int m = e / segs, seg = e - (e / segs) * segs;
-            const int k = kt * WG_BK + m;
-            uint16_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-            if (e < dchunks && seg < p.nseg && k < p.K) {
-                const int oh = oh0 + seg / p.segrow;
-                const int ow0 = (seg % p.segrow) * 8;
-                if (oh < p.OH) {
-                    const uint16_t *src = dy + ((long long)n * p.K + k) * OHW + (long long)oh * p.OW + ow0;
+        for (int q = 0; q < NB; ++q) v[q] = dy_chunk<ALIGNED>(dy, p, e0 + q * WG_THREADS + tid, n, oh0, kt, segs, OHW);
 #pragma unroll
-                    for (int t = 0; t < 8; ++t)
-                        if (ow0 + t < p.OW) h[t] = src[t];
-                }
-            }
-            v[q] = pack8(h);
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < NB; ++q) {
             const int e = e0 + q * WG_THREADS + tid;
             if (e < dchunks) {
                 const int m = e / segs, seg = e - (e / segs) * segs;
@@ -170,29 +121,12 @@ __device__ inline void load_direct(const uint16_t *__restrict__ x, const uint16_
     }
     const int qrow = p.PWpad / 8;
     const int xchunks = WG_BC * p.PR * qrow;
-    for (int e0 = 0; e0 < xchunks; e0 += 4 * WG_THREADS) {
-        uint4 v[4];
+    for (int e0 = 0; e0 < xchunks; e0 += NB * WG_THREADS) {
+        uint4 v[NB];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int e = e0 + q * WG_THREADS + tid;
-            const int cr = e / qrow, qq = e - (e / qrow) * qrow;
-            const int c = cr / p.PR, r = cr - (cr / p.PR) * p.PR;
-            const int ch = ct * WG_BC + c;
-            const int ih = oh0 * p.s1 - p.p1 + r;
-            const int iw0 = qq * 8 - p.PADL;
-            uint16_t h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-            if (e < xchunks && ch < p.C && ih >= 0 && ih < p.H) {
-                const uint16_t *src = x + ((long long)n * p.C + ch) * HW + (long long)ih * p.W;
+        for (int q = 0; q < NB; ++q) v[q] = x_chunk<ALIGNED>(x, p, e0 + q * WG_THREADS + tid, n, oh0, ct, qrow, xchunks, HW);
 #pragma unroll
-                for (int t = 0; t < 8; ++t) {
-                    const int iw = iw0 + t;
-                    if (iw >= 0 && iw < p.W) h[t] = src[iw];
-                }
-            }
-            v[q] = pack8(h);
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < NB; ++q) {
             const int e = e0 + q * WG_THREADS + tid;
             if (e < xchunks) {
                 const int cr = e / qrow, qq = e - (e / qrow) * qrow;
@@ -252,7 +186,7 @@ __device__ inline void b_frags(const uint16_t *rowp, int e0, wu32x4 (&bf)[FW]) {
 }
 
 template <int FW, int FH, int S0, int F16, bool ALIGNED>
-__global__ __launch_bounds__(WG_THREADS, 1) void wgrad_rows(float *__restrict__ out,
+__global__ __launch_bounds__(WG_THREADS, 2) void wgrad_rows(float *__restrict__ out,
                                                             const uint16_t *__restrict__ x,
                                                             const uint16_t *__restrict__ dy,
                                                             WGParams p) {
@@ -278,24 +212,11 @@ __global__ __launch_bounds__(WG_THREADS, 1) void wgrad_rows(float *__restrict__ 
     const int cl = wave * 16 + r16;  // lane's channel within the tile
     const int u0 = split * p.upsplit;
     const int u1 = min(u0 + p.upsplit, p.total_units);
-    BlockRegs R;
-    if (ALIGNED && u0 < u1) {
-        const int n = u0 / p.units_per_img;
-        fetch_block(R, x, dy, p, n, (u0 - n * p.units_per_img) * p.R_oh, kt, ct);
-    }
     for (int u = u0; u < u1; ++u) {
+        const int n = u / p.units_per_img;
         __syncthreads();  // the previous block is no longer read
-        if constexpr (ALIGNED) {
-            commit_block(R, lds_dy, lds_x, p);
-        } else {
-            const int n = u / p.units_per_img;
-            load_direct(x, dy, lds_dy, lds_x, p, n, (u - n * p.units_per_img) * p.R_oh, kt, ct);
-        }
+        load_direct<ALIGNED>(x, dy, lds_dy, lds_x, p, n, (u - n * p.units_per_img) * p.R_oh, kt, ct);
         __syncthreads();
-        if (ALIGNED && u + 1 < u1) {  // next block's loads overlap this block's MFMAs
-            const int n1 = (u + 1) / p.units_per_img;
-            fetch_block(R, x, dy, p, n1, (u + 1 - n1 * p.units_per_img) * p.R_oh, kt, ct);
-        }
         for (int s = 0; s < p.steps; ++s) {
             int seg = s * 4 + g;
             // A fragments: dy rows mi*16 + r16, pixels seg*8 .. +7
@@ -347,15 +268,30 @@ __global__ __launch_bounds__(WG_THREADS, 1) void wgrad_rows(float *__restrict__ 
         }
 }
 
-// Sums nsplit partial dW buffers in split order and converts to the output
-// type (deterministic: the order never depends on scheduling).
+// Sums nsplit partial dW buffers and converts to the output type. The order
+// of the additions is fixed (groups of 8 splits, then the groups in order),
+// so results do not depend on scheduling. Eight partials are in flight per
+// lane.
+__device__ inline float sum_splits(const float *__restrict__ part, long long total, long long e,
+                                   int nsplit) {
+    float s = 0.f;
+    int q = 0;
+    for (; q + 8 <= nsplit; q += 8) {
+        float v[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) v[t] = part[(long long)(q + t) * total + e];
+        s += ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+    }
+    for (; q < nsplit; ++q) s += part[(long long)q * total + e];
+    return s;
+}
+
 template <typename TOut>
 __global__ __launch_bounds__(256) void wgrad_sum(TOut *__restrict__ out, const float *__restrict__ part,
                                                  long long total, int nsplit) {
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
          e += (long long)gridDim.x * blockDim.x) {
-        float s = 0.f;
-        for (int q = 0; q < nsplit; ++q) s += part[(long long)q * total + e];
+        const float s = sum_splits(part, total, e, nsplit);
         if constexpr (sizeof(TOut) == 4)
             out[e] = s;
         else
@@ -366,11 +302,8 @@ __global__ __launch_bounds__(256) void wgrad_sum_f16(_Float16 *__restrict__ out,
                                                      const float *__restrict__ part, long long total,
                                                      int nsplit) {
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
-         e += (long long)gridDim.x * blockDim.x) {
-        float s = 0.f;
-        for (int q = 0; q < nsplit; ++q) s += part[(long long)q * total + e];
-        out[e] = (_Float16)s;
-    }
+         e += (long long)gridDim.x * blockDim.x)
+        out[e] = (_Float16)sum_splits(part, total, e, nsplit);
 }
 
 static size_t wg_lds(const WGParams &p) {
@@ -401,9 +334,7 @@ static bool wg_plan(WGParams &p, int fw, int s0, int d0) {
         p.steps = (p.nseg + 3) / 4;
         p.dyp = p.steps * 32 + 8;
         p.PR = (r - 1) * p.s1 + (p.fh - 1) * p.d1 + 1;
-        const bool regs_ok = WG_BK * p.steps * 4 <= WG_THREADS * WG_MAXD &&
-                             WG_BC * p.PR * (p.PWpad / 8) <= WG_THREADS * WG_MAXX;
-        if (wg_lds(p) <= WG_LDS_LIMIT && regs_ok) break;
+        if (wg_lds(p) <= WG_LDS_LIMIT) break;
     }
     if (r < 1) return false;
     p.units_per_img = (p.OH + p.R_oh - 1) / p.R_oh;

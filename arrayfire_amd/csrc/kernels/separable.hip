@@ -95,13 +95,7 @@ __global__ __launch_bounds__(256) void sep_fused_f32(float *__restrict__ out,
 
     const dim_t ox0 = bx * B, oy0 = by * B;
     const dim_t ix0 = ox0 + p.sx - (cf - 1), iy0 = oy0 + p.sy - (rf - 1);
-    for (int idx = threadIdx.x; idx < LW * LH; idx += 256) {
-        const int r = idx / LW, c = idx - r * LW;
-        const dim_t gx = ix0 + c, gy = iy0 + r;
-        float v = 0.f;
-        if (gx >= 0 && gx < p.S0 && gy >= 0 && gy < p.S1) v = in[gy * p.S0 + gx];
-        xin[r * LW + c] = v;
-    }
+    load_halo_tile_f32<256, (LWMAX + 6) / 4>(xin, in, p.S0, p.S1, ix0, iy0, LH, LW, LW);
     __syncthreads();
 
     // Column pass: tmid(x, r) = sum_k xin(x + k, r) * wc[k] (flipped taps).

@@ -133,13 +133,7 @@ __global__ __launch_bounds__(256) void conv2_tile(float *__restrict__ out,
 
     const dim_t ox0 = bx * BX, oy0 = by * BY;
     const dim_t ix0 = ox0 + p.sx - (FX - 1), iy0 = oy0 + p.sy - (FY - 1);
-    for (int idx = threadIdx.x; idx < LH * LW; idx += 256) {
-        const int r = idx / LW, c = idx - r * LW;
-        const dim_t gx = ix0 + c, gy = iy0 + r;
-        float v = 0.f;
-        if (gx >= 0 && gx < p.S0 && gy >= 0 && gy < p.S1) v = in[gy * p.S0 + gx];
-        tile[r * LS + c] = v;
-    }
+    load_halo_tile_f32<256, (LW + 6) / 4>(tile, in, p.S0, p.S1, ix0, iy0, LH, LW, LS);
     __syncthreads();
 
     const int tx = threadIdx.x % NTX, ty = threadIdx.x / NTX;

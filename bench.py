@@ -202,6 +202,66 @@ def cpu_baseline(budget_s=15.0):
                       "im2col/col2im single-threaded C, SGEMM on numpy's OpenBLAS"}
 
 
+def spatial_secondary(af, ev, stream, reps=10):
+    """FP32 spatial configurations (reported beside the headline): the
+    reference CPU-backend semantics, timed per call with HIP events on the
+    library stream; HBM roofline from algorithmic bytes (each input and output
+    element once)."""
+    import oracle
+
+    rng = np.random.default_rng(20261202 + 1)
+    res = {}
+
+    def timeit(fn):
+        fn()
+        af.sync()
+        e0, e1 = ev.create(2)
+        ev.record(e0, stream)
+        hold = None
+        for _ in range(reps):
+            hold = fn()
+        ev.record(e1, stream)
+        ms = ev.ms(e0, e1) / reps
+        del hold
+        return ms
+
+    def entry(name, ms, nbytes, nflops, npix, extra=None):
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        d = {"ms": round(ms, 4), "GB_s": round(gbs, 1), "hbm_frac": round(gbs / PEAK["hbm_GBs"], 3),
+             "TFLOP_s": round(nflops / (ms * 1e-3) / 1e12, 2),
+             "GPix_s": round(npix / (ms * 1e-3) / 1e9, 2)}
+        if extra:
+            d.update(extra)
+        res[name] = d
+
+    # 2a: 256 x 1024^2, 7x7, AF_BATCH_LHS
+    x = af.Array(rng.random((1024, 1024, 256), dtype=np.float32))
+    w7 = rng.standard_normal((7, 7)).astype(np.float32)
+    W7 = af.Array(w7)
+    ms = timeit(lambda: af.convolve2(x, W7, af.AF_CONV_DEFAULT, af.AF_CONV_SPATIAL))
+    n = 1024 * 1024 * 256
+    entry("2a_conv2_lhs_256x1024sq_7x7", ms, 8.0 * n + 4 * 49, 2.0 * 49 * n, n)
+    # 2b: separable 17+17 Gaussian sigma 3 on the same images
+    g = oracle.gaussian_kernel(17, 1, 3.0).ravel()
+    G = af.Array(g)
+    ms = timeit(lambda: af.convolve_sep(G, G, x, af.AF_CONV_DEFAULT))
+    entry("2b_sep_17x17_256x1024sq", ms, 8.0 * n + 8 * 17, 2.0 * 34 * n, n)
+    del x
+    # 2c: 128^3 volume, 3x3x3
+    v = af.Array(rng.random((128, 128, 128), dtype=np.float32))
+    w3 = af.Array(rng.uniform(-1, 1, (3, 3, 3)).astype(np.float32))
+    ms = timeit(lambda: af.convolve3(v, w3, af.AF_CONV_DEFAULT, af.AF_CONV_SPATIAL))
+    n3 = 128 ** 3
+    entry("2c_conv3_128cube_3x3x3", ms, 8.0 * n3, 2.0 * 27 * n3, n3, {"note": "16 MiB working set: L2/MALL resident"})
+    # 1: 512^2, 5x5
+    s1 = af.Array(rng.random((512, 512), dtype=np.float32))
+    w5 = af.Array(rng.uniform(-1, 1, (5, 5)).astype(np.float32))
+    ms = timeit(lambda: af.convolve2(s1, w5, af.AF_CONV_DEFAULT, af.AF_CONV_SPATIAL))
+    n1 = 512 * 512
+    entry("1_conv2_512sq_5x5", ms, 8.0 * n1, 2.0 * 25 * n1, n1, {"note": "2 MiB working set: cache resident"})
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -209,6 +269,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 error check")
+    ap.add_argument("--no-spatial", action="store_true", help="skip the spatial secondary lines")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -295,6 +356,8 @@ def main():
                                                          zip(names, errs)},
         "roofline": roof,
     }
+    if not args.no_spatial:
+        out["secondary_fp32_spatial"] = spatial_secondary(af, ev, stream)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))
