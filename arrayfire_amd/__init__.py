@@ -75,6 +75,9 @@ EXPORTED = [
     "afhip_k_conv2_nn_wgrad_workspace", "afhip_k_conv2_nn_wgrad",
     "afhip_k_conv2_nn_f64", "afhip_k_conv2_nn_patch_workspace", "afhip_k_conv2_nn_patch",
     "afhip_k_conv2_nn_wgrad_rows_workspace", "afhip_k_conv2_nn_wgrad_rows",
+    "afhip_ragged_plan_create", "afhip_ragged_plan_run", "afhip_ragged_plan_variant",
+    "afhip_ragged_plan_release", "afhip_convolve2_ragged", "afhip_k_ragged_tile",
+    "afhip_k_convolve2_ragged",
 ]
 
 
@@ -151,6 +154,14 @@ def lib() -> ctypes.CDLL:
         L.afhip_k_conv2_nn_wgrad_rows.argtypes = [ctypes.c_int] * 3 + [vp, vp, vp] + [i64p] * 6 + [vp, ll, vp]
         L.afhip_k_conv2_nn_f64.argtypes = [ctypes.c_int, vp, vp, vp] + [i64p] * 6 + [vp]
         L.afhip_k_scale.argtypes = [ctypes.c_int, vp, vp, ctypes.c_double, ll, vp]
+        L.afhip_ragged_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                               ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                               ctypes.POINTER(ctypes.c_int), u]
+        L.afhip_ragged_plan_run.argtypes = [vp]
+        L.afhip_ragged_plan_variant.argtypes = [ctypes.POINTER(ctypes.c_int), vp]
+        L.afhip_ragged_plan_release.argtypes = [vp]
+        L.afhip_convolve2_ragged.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                             ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_int), u]
         _lib = L
     return _lib
 
@@ -352,6 +363,53 @@ def convolve2GradientNN(incoming_gradient, original_signal, original_filter, con
     return _out(lib().afhip_convolve2_gradient_nn_ex, g.handle, s.handle, f.handle, o.handle,
                 len(stride), st, len(padding), pd, len(dilation), dl, grad_type,
                 precision or 0, -1 if out_type is None else out_type)
+
+
+class RaggedPlan:
+    """afhip_ragged_plan: a captured persistent launch over many independent
+    af_convolve2(signal, filter, mode, AF_CONV_SPATIAL) problems."""
+
+    def __init__(self, signals, filters, modes):
+        self._sig = [_arr(s) for s in signals]
+        self._flt = [_arr(f) for f in filters]
+        n = len(self._sig)
+        if len(self._flt) != n or len(modes) != n:
+            raise ValueError("signals, filters and modes differ in length")
+        VP = ctypes.c_void_p * max(n, 1)
+        outs = VP()
+        sigs = VP(*[a.handle.value for a in self._sig])
+        flts = VP(*[a.handle.value for a in self._flt])
+        md = (ctypes.c_int * max(n, 1))(*modes)
+        self._plan = ctypes.c_void_p()
+        check(lib().afhip_ragged_plan_create(ctypes.byref(self._plan), outs, sigs, flts, md, n))
+        self.outputs = [Array(handle=outs[i]) for i in range(n)]
+
+    def run(self) -> None:
+        check(lib().afhip_ragged_plan_run(self._plan))
+
+    @property
+    def variant(self) -> int:
+        v = ctypes.c_int()
+        check(lib().afhip_ragged_plan_variant(ctypes.byref(v), self._plan))
+        return v.value
+
+    def __del__(self):
+        if _lib is not None and getattr(self, "_plan", None) and self._plan.value:
+            _lib.afhip_ragged_plan_release(self._plan)
+            self._plan = ctypes.c_void_p()
+
+
+def convolve2_ragged(signals, filters, modes) -> list:
+    """One-shot afhip_convolve2_ragged: returns the output Arrays."""
+    sig = [_arr(s) for s in signals]
+    flt = [_arr(f) for f in filters]
+    n = len(sig)
+    VP = ctypes.c_void_p * max(n, 1)
+    outs = VP()
+    check(lib().afhip_convolve2_ragged(outs, VP(*[a.handle.value for a in sig]),
+                                       VP(*[a.handle.value for a in flt]),
+                                       (ctypes.c_int * max(n, 1))(*modes), n))
+    return [Array(handle=outs[i]) for i in range(n)]
 
 
 def filter(image, kernel) -> Array:  # noqa: A001 - af::filter

@@ -1,0 +1,118 @@
+// Project autotuner: picks one of several kernel variants for a key (shape,
+// filter, dtype, ...) by timing each candidate once on the library stream,
+// and remembers the choice. Choices persist across processes in a JSON file:
+// $AFHIP_AUTOTUNE_CACHE if set, else $HOME/.cache/afhip/autotune.json; set
+// AFHIP_AUTOTUNE_CACHE=off to keep them in memory only.
+#include "autotune.hpp"
+
+#include <sys/stat.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
+
+namespace afhip {
+
+namespace {
+std::mutex g_m;
+std::map<std::string, int> *g_cache = nullptr;
+bool g_loaded = false;
+
+std::string cache_path() {
+    const char *e = std::getenv("AFHIP_AUTOTUNE_CACHE");
+    if (e) return std::string(e) == "off" ? std::string() : std::string(e);
+    const char *h = std::getenv("HOME");
+    if (!h) return std::string();
+    return std::string(h) + "/.cache/afhip/autotune.json";
+}
+
+// Minimal reader for the flat {"key": int, ...} object this file writes.
+void load_locked() {
+    if (g_loaded) return;
+    g_loaded = true;
+    if (!g_cache) g_cache = new std::map<std::string, int>();
+    const std::string path = cache_path();
+    if (path.empty()) return;
+    std::ifstream f(path);
+    if (!f) return;
+    std::stringstream ss;
+    ss << f.rdbuf();
+    const std::string s = ss.str();
+    size_t i = 0;
+    while (true) {
+        const size_t k0 = s.find('"', i);
+        if (k0 == std::string::npos) break;
+        const size_t k1 = s.find('"', k0 + 1);
+        if (k1 == std::string::npos) break;
+        const size_t c = s.find(':', k1);
+        if (c == std::string::npos) break;
+        const int v = std::atoi(s.c_str() + c + 1);
+        (*g_cache)[s.substr(k0 + 1, k1 - k0 - 1)] = v;
+        i = c + 1;
+    }
+}
+
+void save_locked() {
+    const std::string path = cache_path();
+    if (path.empty()) return;
+    const size_t slash = path.rfind('/');
+    if (slash != std::string::npos) {
+        std::string dir = path.substr(0, slash);
+        // create parent directories (best effort)
+        for (size_t p = 1; p <= dir.size(); ++p)
+            if (p == dir.size() || dir[p] == '/') mkdir(dir.substr(0, p).c_str(), 0755);
+    }
+    const std::string tmp = path + ".tmp";
+    std::ofstream f(tmp);
+    if (!f) return;
+    f << "{\n";
+    size_t n = 0;
+    for (const auto &kv : *g_cache) f << "  \"" << kv.first << "\": " << kv.second << (++n < g_cache->size() ? ",\n" : "\n");
+    f << "}\n";
+    f.close();
+    std::rename(tmp.c_str(), path.c_str());
+}
+}  // namespace
+
+int autotune_lookup(const std::string &key) {
+    std::lock_guard<std::mutex> g(g_m);
+    load_locked();
+    auto it = g_cache->find(key);
+    return it == g_cache->end() ? -1 : it->second;
+}
+
+int autotune_pick(const std::string &key, int ncand, const std::function<void(int)> &launch) {
+    const int known = autotune_lookup(key);
+    if (known >= 0 && known < ncand) return known;
+    hipStream_t s = stream();
+    hipEvent_t e0, e1;
+    check_hip(hipEventCreate(&e0), "hipEventCreate");
+    check_hip(hipEventCreate(&e1), "hipEventCreate");
+    int best = 0;
+    float best_ms = 1e30f;
+    for (int c = 0; c < ncand; ++c) {
+        launch(c);  // warm-up (code load, caches)
+        check_hip(hipEventRecord(e0, s), "hipEventRecord");
+        for (int r = 0; r < 3; ++r) launch(c);
+        check_hip(hipEventRecord(e1, s), "hipEventRecord");
+        check_hip(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0.f;
+        check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::lock_guard<std::mutex> g(g_m);
+    load_locked();
+    (*g_cache)[key] = best;
+    save_locked();
+    return best;
+}
+
+}  // namespace afhip

@@ -262,6 +262,54 @@ def spatial_secondary(af, ev, stream, reps=10):
     return res
 
 
+def ragged_secondary(af, ev, stream):
+    """cfg 5: 2000 independent spatial 2-D convolutions (committed list in
+    tests/golden/ragged_problems.json) as one graph-captured persistent
+    launch, next to looping af_convolve2 over the same problems."""
+    with open(os.path.join(ROOT, "tests", "golden", "ragged_problems.json")) as f:
+        probs = json.load(f)["problems"]
+    rng = np.random.default_rng(20261202 + 4)
+    sig, flt, modes = [], [], []
+    flops = bytes_ = 0.0
+    for w_, h_, fw, fh, m in probs:
+        sig.append(af.Array(rng.random((w_, h_), dtype=np.float32)))
+        flt.append(af.Array(rng.random((fw, fh), dtype=np.float32)))
+        modes.append(m)
+        ow, oh = (w_ + fw - 1, h_ + fh - 1) if m else (w_, h_)
+        flops += 2.0 * fw * fh * ow * oh
+        bytes_ += 4.0 * (w_ * h_ + ow * oh)
+    t0 = time.perf_counter()
+    plan = af.RaggedPlan(sig, flt, modes)
+    setup_s = time.perf_counter() - t0
+    plan.run()
+    af.sync()
+    e0, e1 = ev.create(2)
+    reps = 5
+    ev.record(e0, stream)
+    for _ in range(reps):
+        plan.run()
+    ev.record(e1, stream)
+    ms_graph = ev.ms(e0, e1) / reps
+    # per-call loop (host dispatch + one launch per problem)
+    outs = [af.convolve2(s_, f_, m, af.AF_CONV_SPATIAL) for s_, f_, m in zip(sig, flt, modes)]
+    af.sync()
+    del outs
+    t0 = time.perf_counter()
+    ev.record(e0, stream)
+    outs = [af.convolve2(s_, f_, m, af.AF_CONV_SPATIAL) for s_, f_, m in zip(sig, flt, modes)]
+    ev.record(e1, stream)
+    ms_loop = ev.ms(e0, e1)
+    loop_wall = (time.perf_counter() - t0) * 1e3
+    del outs, plan
+    return {"problems": len(probs), "ms_graph": round(ms_graph, 3), "ms_per_call_loop": round(ms_loop, 3),
+            "ms_per_call_loop_host_wall": round(loop_wall, 3), "speedup": round(ms_loop / ms_graph, 2),
+            "TFLOP_s": round(flops / (ms_graph * 1e-3) / 1e12, 2),
+            "GB_s": round(bytes_ / (ms_graph * 1e-3) / 1e9, 1),
+            "hbm_frac": round(bytes_ / (ms_graph * 1e-3) / 1e9 / PEAK["hbm_GBs"], 3),
+            "algorithmic_flops": flops, "algorithmic_bytes": bytes_,
+            "plan_setup_s_incl_autotune": round(setup_s, 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -358,6 +406,7 @@ def main():
     }
     if not args.no_spatial:
         out["secondary_fp32_spatial"] = spatial_secondary(af, ev, stream)
+        out["secondary_fp32_ragged"] = ragged_secondary(af, ev, stream)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

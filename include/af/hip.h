@@ -71,3 +71,28 @@ AFAPI af_err afhip_convolve2_gradient_nn_ex(
     const unsigned dilation_dims, const dim_t *dilations,
     af_conv_gradient_type grad_type, afhip_precision precision,
     af_dtype out_type);
+
+/* Ragged batches: count independent 2-D spatial convolutions, each exactly
+ * af_convolve2(&outputs[i], signals[i], filters[i], modes[i],
+ * AF_CONV_SPATIAL) for an f32 signal with exactly 2 dims (dim1 > 1) and a
+ * 2-D filter of at most 15x15
+ * (any dtype, converted to f32 as the reference's castArray does), run as one
+ * persistent kernel over a cost-sorted (problem, tile) work list. The plan
+ * allocates the outputs (new handles in outputs[], owned by the caller),
+ * picks the tile shape with the autotuner (cached per problem list), and
+ * captures the launch in a hipGraph; afhip_ragged_plan_run replays it on the
+ * library stream (inputs are read again, so new signal values are picked
+ * up). No counterpart in the reference. */
+typedef struct afhip_ragged_plan_t *afhip_ragged_plan;
+AFAPI af_err afhip_ragged_plan_create(afhip_ragged_plan *plan, af_array *outputs,
+                                      const af_array *signals,
+                                      const af_array *filters,
+                                      const af_conv_mode *modes, unsigned count);
+AFAPI af_err afhip_ragged_plan_run(afhip_ragged_plan plan);
+/* The tile variant the autotuner chose (0: 128x64, 1: 64x64, 2: 64x32). */
+AFAPI af_err afhip_ragged_plan_variant(int *variant, const afhip_ragged_plan plan);
+AFAPI af_err afhip_ragged_plan_release(afhip_ragged_plan plan);
+/* One-shot: create, run once, release (outputs stay valid). */
+AFAPI af_err afhip_convolve2_ragged(af_array *outputs, const af_array *signals,
+                                    const af_array *filters,
+                                    const af_conv_mode *modes, unsigned count);

@@ -167,3 +167,15 @@ AFHIP_KAPI int afhip_k_conv2_nn_wgrad_rows(int in_type, int out_type,
                                            void *workspace,
                                            long long workspace_bytes,
                                            void *stream);
+
+/* Ragged batch of FP32 2-D spatial convolutions (afhip_ragged_plan_* in
+ * include/af/hip.h): problems is a device array of per-problem descriptors
+ * {const float *in; float *out; const float *filt; int S0, S1, O0, O1, FX,
+ * FY, sx, sy, nbx, nby;} and items a device array of nitems (problem, tile)
+ * int pairs; grid persistent workgroups process items round-robin. Filters
+ * up to 15x15. afhip_k_ragged_tile returns the output-tile edges of a
+ * variant (nbx/nby are the problem's tile counts for that variant). */
+AFHIP_KAPI int afhip_k_ragged_tile(int variant, int *bx, int *by);
+AFHIP_KAPI int afhip_k_convolve2_ragged(int variant, const void *problems,
+                                        const void *items, long long nitems,
+                                        int grid, void *stream);

@@ -221,6 +221,15 @@ def convolve(signal, filt, rank: int, expand: bool) -> np.ndarray:
     return out.reshape(od, order="F")
 
 
+def convolve_dispatch(signal, filt, rank: int, expand: bool) -> np.ndarray:
+    """af_convolve{rank} with AF_CONV_SPATIAL: drops to a lower rank while
+    either operand has fewer dims (src/api/c/convolve.cpp:231-234, 246-249),
+    then convolve()."""
+    while rank > 1 and (ndims(dims4(signal)) < rank or ndims(dims4(filt)) < rank):
+        rank -= 1
+    return convolve(signal, filt, rank, expand)
+
+
 def convolve2_sep(col_filter, row_filter, signal, expand: bool) -> np.ndarray:
     """Separable 2-D convolution, src/backend/cpu/convolve.cpp:68-99 and
     src/backend/cpu/kernel/convolve.hpp:213-289 (InT-typed temp).
