@@ -77,7 +77,8 @@ EXPORTED = [
     "afhip_k_conv2_nn_wgrad_rows_workspace", "afhip_k_conv2_nn_wgrad_rows",
     "afhip_ragged_plan_create", "afhip_ragged_plan_run", "afhip_ragged_plan_variant",
     "afhip_ragged_plan_release", "afhip_convolve2_ragged", "afhip_k_ragged_tile",
-    "afhip_k_convolve2_ragged",
+    "afhip_k_convolve2_ragged", "afhip_k_conv2_nn_patch_pack", "afhip_k_conv2_nn_patch_run",
+    "afhip_k_conv2_nn_wgrad_rows_partial", "afhip_k_conv2_nn_wgrad_rows_reduce",
 ]
 
 
@@ -152,6 +153,12 @@ def lib() -> ctypes.CDLL:
         L.afhip_k_conv2_nn_wgrad_rows_workspace.argtypes = [ctypes.c_int] * 2 + [i64p] * 6
         L.afhip_k_conv2_nn_wgrad_rows_workspace.restype = ll
         L.afhip_k_conv2_nn_wgrad_rows.argtypes = [ctypes.c_int] * 3 + [vp, vp, vp] + [i64p] * 6 + [vp, ll, vp]
+        L.afhip_k_conv2_nn_patch_pack.argtypes = [ctypes.c_int] * 3 + [vp] + [i64p] * 6 + [vp, ll, vp]
+        L.afhip_k_conv2_nn_patch_run.argtypes = [ctypes.c_int] * 4 + [vp, vp] + [i64p] * 6 + [vp, ll, vp]
+        L.afhip_k_conv2_nn_wgrad_rows_partial.argtypes = [ctypes.c_int] * 2 + [vp, vp] + [i64p] * 6 + [vp, ll, vp]
+        L.afhip_k_conv2_nn_wgrad_rows_reduce.argtypes = [ctypes.c_int] * 3 + [vp] + [i64p] * 6 + [vp, ll, vp]
+        L.afhip_k_ragged_tile.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.afhip_k_convolve2_ragged.argtypes = [ctypes.c_int, vp, vp, ll, ctypes.c_int, vp]
         L.afhip_k_conv2_nn_f64.argtypes = [ctypes.c_int, vp, vp, vp] + [i64p] * 6 + [vp]
         L.afhip_k_scale.argtypes = [ctypes.c_int, vp, vp, ctypes.c_double, ll, vp]
         L.afhip_ragged_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(vp),

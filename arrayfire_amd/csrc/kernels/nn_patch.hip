@@ -120,7 +120,9 @@ __global__ __launch_bounds__(256) void pack_weights(void *__restrict__ out,
 }
 
 // ---------------------------------------------------------------- main kernel
-template <int MODE, typename TIn, typename TOut>
+// PASS (0 forward, 1 data gradient) changes nothing in the code; it gives the
+// two uses distinct kernel names in profiles.
+template <int MODE, typename TIn, typename TOut, int PASS>
 __global__ __launch_bounds__(P_THREADS, 4) void conv_patch(TOut *__restrict__ y,
                                                         const TIn *__restrict__ x,
                                                         const char *__restrict__ wpk,
@@ -405,29 +407,35 @@ static bool plan(PatchParams &p) {
     return lds_bytes<MODE>(p) <= kPatchLdsLimit && p.PW > 0 && p.PRmax > 0;
 }
 
+// what: 1 = pack the weights into ws, 2 = run the convolution with the
+// weights already packed in ws, 3 = both.
 template <int MODE, typename TIn, typename TOut>
 static int launch(void *y, const void *x, const void *w, PatchParams &p, int dgrad, void *ws,
-                  hipStream_t s) {
+                  int what, hipStream_t s) {
     using Cfg = PCfg<MODE>;
-    const long long plane = (long long)p.ktiles * p.nchunks * p.RS * P_BM * Cfg::KC;
-    // weights: K x C x fh x fw of the original filter; for dgrad the GEMM's
-    // M is the original C and its channels are the original K
-    const int wC = dgrad ? p.K : p.C;
-    long long blocks = (plane + 255) / 256;
-    if (blocks > kMaxGridBlocks) blocks = kMaxGridBlocks;
-    pack_weights<MODE, TIn><<<(unsigned)blocks, 256, 0, s>>>(ws, (const TIn *)w, p.K, p.C, p.fw, p.fh,
-                                                               wC, p.nchunks, p.ktiles, dgrad);
-    const size_t lds = lds_bytes<MODE>(p);
-    auto fn = conv_patch<MODE, TIn, TOut>;
-    static bool attr_set = false;
-    if (!attr_set) {
-        if (hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)kPatchLdsLimit) != hipSuccess)
-            return K_ERR_RUNTIME;
-        attr_set = true;
+    if (what & 1) {
+        const long long plane = (long long)p.ktiles * p.nchunks * p.RS * P_BM * Cfg::KC;
+        // weights: K x C x fh x fw of the original filter; for dgrad the
+        // GEMM's M is the original C and its channels are the original K
+        const int wC = dgrad ? p.K : p.C;
+        long long blocks = (plane + 255) / 256;
+        if (blocks > kMaxGridBlocks) blocks = kMaxGridBlocks;
+        pack_weights<MODE, TIn><<<(unsigned)blocks, 256, 0, s>>>(ws, (const TIn *)w, p.K, p.C, p.fw,
+                                                                   p.fh, wC, p.nchunks, p.ktiles, dgrad);
     }
-    fn<<<dim3((unsigned)p.ntiles), P_THREADS, lds, s>>>((TOut *)y, (const TIn *)x,
-                                                          (const char *)ws, p);
+    if (what & 2) {
+        const size_t lds = lds_bytes<MODE>(p);
+        auto fn = dgrad ? conv_patch<MODE, TIn, TOut, 1> : conv_patch<MODE, TIn, TOut, 0>;
+        static bool attr_set[2] = {false, false};
+        if (!attr_set[dgrad]) {
+            if (hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)kPatchLdsLimit) != hipSuccess)
+                return K_ERR_RUNTIME;
+            attr_set[dgrad] = true;
+        }
+        fn<<<dim3((unsigned)p.ntiles), P_THREADS, lds, s>>>((TOut *)y, (const TIn *)x,
+                                                              (const char *)ws, p);
+    }
     return launch_status();
 }
 
@@ -494,11 +502,10 @@ extern "C" long long afhip_k_conv2_nn_patch_workspace(int pass, int in_type, int
     return (long long)p.ktiles * p.nchunks * p.RS * P_BM * kc;
 }
 
-extern "C" int afhip_k_conv2_nn_patch(int pass, int in_type, int out_type, int precision, void *out,
-                                      const void *in, const void *w, const long long *xd,
-                                      const long long *wd, const long long *yd, const long long *st,
-                                      const long long *pd, const long long *dl, void *workspace,
-                                      long long workspace_bytes, void *stream) {
+static int patch_call(int what, int pass, int in_type, int out_type, int precision, void *out,
+                      const void *in, const void *w, const long long *xd, const long long *wd,
+                      const long long *yd, const long long *st, const long long *pd,
+                      const long long *dl, void *workspace, long long workspace_bytes, void *stream) {
     const long long need = afhip_k_conv2_nn_patch_workspace(pass, in_type, precision, xd, wd, yd, st, pd, dl);
     if (need == 0) return K_ERR_NOT_SUPPORTED;
     if (workspace == nullptr || workspace_bytes < need) return K_ERR_ARG;
@@ -512,7 +519,7 @@ extern "C" int afhip_k_conv2_nn_patch(int pass, int in_type, int out_type, int p
         case PM_F16: plan<PM_F16>(p); break;
         default: plan<PM_SPLIT>(p); break;
     }
-#define GO(MD, TI, TO) return launch<MD, TI, TO>(out, in, w, p, pass, workspace, s)
+#define GO(MD, TI, TO) return launch<MD, TI, TO>(out, in, w, p, pass, workspace, what, s)
     if (in_type == DT_F32 && out_type == DT_F32) {
         switch (mode) {
             case PM_F32: GO(PM_F32, float, float);
@@ -531,4 +538,33 @@ extern "C" int afhip_k_conv2_nn_patch(int pass, int in_type, int out_type, int p
     }
 #undef GO
     return K_ERR_TYPE;
+}
+
+extern "C" int afhip_k_conv2_nn_patch(int pass, int in_type, int out_type, int precision, void *out,
+                                      const void *in, const void *w, const long long *xd,
+                                      const long long *wd, const long long *yd, const long long *st,
+                                      const long long *pd, const long long *dl, void *workspace,
+                                      long long workspace_bytes, void *stream) {
+    return patch_call(3, pass, in_type, out_type, precision, out, in, w, xd, wd, yd, st, pd, dl,
+                      workspace, workspace_bytes, stream);
+}
+
+extern "C" int afhip_k_conv2_nn_patch_pack(int pass, int in_type, int precision, const void *w,
+                                           const long long *xd, const long long *wd,
+                                           const long long *yd, const long long *st,
+                                           const long long *pd, const long long *dl, void *workspace,
+                                           long long workspace_bytes, void *stream) {
+    // the output type does not matter for packing; pass the input's
+    const int ot = in_type == DT_F32 ? DT_F32 : in_type;
+    return patch_call(1, pass, in_type, ot, precision, nullptr, nullptr, w, xd, wd, yd, st, pd, dl,
+                      workspace, workspace_bytes, stream);
+}
+
+extern "C" int afhip_k_conv2_nn_patch_run(int pass, int in_type, int out_type, int precision, void *out,
+                                          const void *in, const long long *xd, const long long *wd,
+                                          const long long *yd, const long long *st, const long long *pd,
+                                          const long long *dl, const void *workspace,
+                                          long long workspace_bytes, void *stream) {
+    return patch_call(2, pass, in_type, out_type, precision, out, in, nullptr, xd, wd, yd, st, pd, dl,
+                      const_cast<void *>(workspace), workspace_bytes, stream);
 }

@@ -400,12 +400,11 @@ extern "C" long long afhip_k_conv2_nn_wgrad_rows_workspace(int in_type, int prec
     return (long long)p.nsplit * p.K * p.C * p.RS * 4;
 }
 
-extern "C" int afhip_k_conv2_nn_wgrad_rows(int in_type, int out_type, int precision, void *dw,
-                                           const void *x, const void *dy, const long long *xd,
-                                           const long long *wd, const long long *yd,
-                                           const long long *st, const long long *pd,
-                                           const long long *dl, void *workspace,
-                                           long long workspace_bytes, void *stream) {
+// what: 1 = partial tiles into the workspace, 2 = sum them into dw, 3 = both
+static int rows_call(int what, int in_type, int out_type, int precision, void *dw, const void *x,
+                     const void *dy, const long long *xd, const long long *wd, const long long *yd,
+                     const long long *st, const long long *pd, const long long *dl, void *workspace,
+                     long long workspace_bytes, void *stream) {
     const long long need = afhip_k_conv2_nn_wgrad_rows_workspace(in_type, precision, xd, wd, yd, st, pd, dl);
     if (need == 0) return K_ERR_NOT_SUPPORTED;
     if (workspace == nullptr || workspace_bytes < need) return K_ERR_ARG;
@@ -416,30 +415,64 @@ extern "C" int afhip_k_conv2_nn_wgrad_rows(int in_type, int out_type, int precis
     p.is_f16 = in_type == DT_F16;
     hipStream_t s = (hipStream_t)stream;
     float *part = (float *)workspace;
-    int rc;
+    if (what & 1) {
+        int rc;
 #define WGO(FWV, FHV, S0V)                                                                    \
     rc = p.is_f16 ? wg_launch<FWV, FHV, S0V, 1>(part, x, dy, p, s)                           \
                   : wg_launch<FWV, FHV, S0V, 0>(part, x, dy, p, s)
 #define WGS(FWV, FHV)            \
     if (s0 == 1) WGO(FWV, FHV, 1); \
     else WGO(FWV, FHV, 2);
-    if (fw == 3 && p.fh == 3) { WGS(3, 3) }
-    else if (fw == 3) { WGS(3, 1) }
-    else if (p.fh == 3) { WGS(1, 3) }
-    else { WGS(1, 1) }
+        if (fw == 3 && p.fh == 3) { WGS(3, 3) }
+        else if (fw == 3) { WGS(3, 1) }
+        else if (p.fh == 3) { WGS(1, 3) }
+        else { WGS(1, 1) }
 #undef WGS
 #undef WGO
-    if (rc) return rc;
-    const long long total = (long long)p.K * p.C * p.RS;
-    long long blocks = (total + 255) / 256;
-    if (blocks > kMaxGridBlocks) blocks = kMaxGridBlocks;
-    if (out_type == DT_F32)
-        wgrad_sum<float><<<(unsigned)blocks, 256, 0, s>>>((float *)dw, part, total, p.nsplit);
-    else if (out_type == DT_BF16 && in_type == DT_BF16)
-        wgrad_sum<uint16_t><<<(unsigned)blocks, 256, 0, s>>>((uint16_t *)dw, part, total, p.nsplit);
-    else if (out_type == DT_F16 && in_type == DT_F16)
-        wgrad_sum_f16<<<(unsigned)blocks, 256, 0, s>>>((_Float16 *)dw, part, total, p.nsplit);
-    else
-        return K_ERR_TYPE;
+        if (rc) return rc;
+    }
+    if (what & 2) {
+        const long long total = (long long)p.K * p.C * p.RS;
+        long long blocks = (total + 255) / 256;
+        if (blocks > kMaxGridBlocks) blocks = kMaxGridBlocks;
+        if (out_type == DT_F32)
+            wgrad_sum<float><<<(unsigned)blocks, 256, 0, s>>>((float *)dw, part, total, p.nsplit);
+        else if (out_type == DT_BF16 && in_type == DT_BF16)
+            wgrad_sum<uint16_t><<<(unsigned)blocks, 256, 0, s>>>((uint16_t *)dw, part, total, p.nsplit);
+        else if (out_type == DT_F16 && in_type == DT_F16)
+            wgrad_sum_f16<<<(unsigned)blocks, 256, 0, s>>>((_Float16 *)dw, part, total, p.nsplit);
+        else
+            return K_ERR_TYPE;
+    }
     return launch_status();
+}
+
+extern "C" int afhip_k_conv2_nn_wgrad_rows(int in_type, int out_type, int precision, void *dw,
+                                           const void *x, const void *dy, const long long *xd,
+                                           const long long *wd, const long long *yd,
+                                           const long long *st, const long long *pd,
+                                           const long long *dl, void *workspace,
+                                           long long workspace_bytes, void *stream) {
+    return rows_call(3, in_type, out_type, precision, dw, x, dy, xd, wd, yd, st, pd, dl, workspace,
+                     workspace_bytes, stream);
+}
+
+extern "C" int afhip_k_conv2_nn_wgrad_rows_partial(int in_type, int precision, const void *x,
+                                                   const void *dy, const long long *xd,
+                                                   const long long *wd, const long long *yd,
+                                                   const long long *st, const long long *pd,
+                                                   const long long *dl, void *workspace,
+                                                   long long workspace_bytes, void *stream) {
+    return rows_call(1, in_type, DT_F32, precision, nullptr, x, dy, xd, wd, yd, st, pd, dl, workspace,
+                     workspace_bytes, stream);
+}
+
+extern "C" int afhip_k_conv2_nn_wgrad_rows_reduce(int in_type, int out_type, int precision, void *dw,
+                                                  const long long *xd, const long long *wd,
+                                                  const long long *yd, const long long *st,
+                                                  const long long *pd, const long long *dl,
+                                                  const void *workspace, long long workspace_bytes,
+                                                  void *stream) {
+    return rows_call(2, in_type, out_type, precision, dw, nullptr, nullptr, xd, wd, yd, st, pd, dl,
+                     const_cast<void *>(workspace), workspace_bytes, stream);
 }

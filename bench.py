@@ -32,6 +32,20 @@ WD = (3, 3, 64, 128)
 ST, PD, DL = (1, 1), (1, 1), (1, 1)
 SEED = 20261202 + 2
 PEAK = {"hbm_GBs": 8000.0, "bf16_TFs": 2500.0, "fp32_TFs": 157.3}
+# Mangled names of the step's kernels (as rocprofv3 reports them).
+KERNEL_SYMBOL = {
+    "conv_fwd": "afhip::conv_patch<1, unsigned short, unsigned short, 0>",
+    "conv_dgrad": "afhip::conv_patch<1, unsigned short, unsigned short, 1>",
+    "wgrad_partial": "afhip::wgrad_rows<3, 3, 1, 0, true>",
+    "wgrad_reduce": "afhip::wgrad_sum<unsigned short>",
+}
+# HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (profiles/),
+# filled in from the committed counter summary when present.
+TRAFFIC = {}
+_tp = os.path.join(ROOT, "profiles", "nn_step_traffic.json")
+if os.path.exists(_tp):
+    with open(_tp) as _f:
+        TRAFFIC = json.load(_f).get("bytes_per_launch", {})
 
 
 def nn_flops(xd, wd, yd):
@@ -68,42 +82,130 @@ class HipEvents:
         return float(t.value)
 
 
-def run_step(af, x, w, dy, y_holder, precision, out_type):
-    y = af.convolve2NN(x, w, ST, PD, DL, precision=precision, out_type=out_type)
-    dx = af.convolve2GradientNN(dy, x, w, y, ST, PD, DL, af.AF_CONV_GRADIENT_DATA,
-                                precision=precision, out_type=out_type)
-    dw = af.convolve2GradientNN(dy, x, w, y, ST, PD, DL, af.AF_CONV_GRADIENT_FILTER,
-                                precision=precision, out_type=out_type)
-    y_holder[:] = [y, dx, dw]
+def _dims(d):
+    return (ctypes.c_longlong * 4)(*d)
 
 
-def time_path(af, ev, stream, x, w, dy, precision, out_type, steps, warmup):
-    hold = []
+def _two(v):
+    return (ctypes.c_longlong * 2)(*v)
+
+
+class NNStep:
+    """One training step of the layer through the kernel C ABI
+    (include/afhip_kernels.h): forward, data gradient, filter gradient, each
+    as its individual kernel launches on the library stream, with outputs and
+    workspaces allocated once. Weights are re-packed every step, as they would
+    be after an optimizer update."""
+
+    def __init__(self, af, X, W, DY, in_type, out_type, precision):
+        self.af, self.L = af, af.lib()
+        self.it, self.ot, self.pr = in_type, out_type, precision
+        self.X, self.W, self.DY = X, W, DY
+        yd = (XD[0], XD[1], WD[3], XD[3])
+        self.xd, self.wd, self.yd = _dims(XD), _dims(WD), _dims(yd)
+        self.st, self.pd, self.dl = _two(ST), _two(PD), _two(DL)
+        L = self.L
+        esz = 4 if out_type == af.f32 else 2
+        self.Y = af.constant(0, yd, out_type)
+        self.DX = af.constant(0, XD, out_type)
+        self.DW = af.constant(0, WD, out_type)
+        g = (self.xd, self.wd, self.yd, self.st, self.pd, self.dl)
+        self.ws_f = L.afhip_k_conv2_nn_patch_workspace(0, in_type, precision, *g)
+        self.ws_d = L.afhip_k_conv2_nn_patch_workspace(1, in_type, precision, *g)
+        self.ws_w = L.afhip_k_conv2_nn_wgrad_rows_workspace(in_type, precision, *g)
+        self.rows = self.ws_w > 0
+        if not self.rows:
+            self.ws_w = L.afhip_k_conv2_nn_wgrad_workspace(*g)
+        assert self.ws_f > 0 and self.ws_d > 0, "patch path expected for this layer"
+        self.WSF = af.constant(0, (max(self.ws_f, 1),), af.u8)
+        self.WSD = af.constant(0, (max(self.ws_d, 1),), af.u8)
+        self.WSW = af.constant(0, (max(self.ws_w, 1),), af.u8)
+        self.g = g
+        self.stream = ctypes.c_void_p(af.stream())
+        del esz
+
+    def kernels(self):
+        """(name, launch) for each kernel of the step, in order."""
+        L, g, s = self.L, self.g, self.stream
+        p = lambda a: ctypes.c_void_p(a.device_ptr())  # noqa: E731
+        it, ot, pr = self.it, self.ot, self.pr
+        ks = [
+            ("pack_fwd", lambda: L.afhip_k_conv2_nn_patch_pack(0, it, pr, p(self.W), *g, p(self.WSF),
+                                                               self.ws_f, s)),
+            ("conv_fwd", lambda: L.afhip_k_conv2_nn_patch_run(0, it, ot, pr, p(self.Y), p(self.X), *g,
+                                                              p(self.WSF), self.ws_f, s)),
+            ("pack_dgrad", lambda: L.afhip_k_conv2_nn_patch_pack(1, it, pr, p(self.W), *g, p(self.WSD),
+                                                                 self.ws_d, s)),
+            ("conv_dgrad", lambda: L.afhip_k_conv2_nn_patch_run(1, it, ot, pr, p(self.DX), p(self.DY), *g,
+                                                                p(self.WSD), self.ws_d, s)),
+        ]
+        if self.rows:
+            ks += [("wgrad_partial", lambda: L.afhip_k_conv2_nn_wgrad_rows_partial(
+                       it, pr, p(self.X), p(self.DY), *g, p(self.WSW), self.ws_w, s)),
+                   ("wgrad_reduce", lambda: L.afhip_k_conv2_nn_wgrad_rows_reduce(
+                       it, ot, pr, p(self.DW), *g, p(self.WSW), self.ws_w, s))]
+        else:
+            ks += [("wgrad", lambda: L.afhip_k_conv2_nn_wgrad(it, ot, pr, p(self.DW), p(self.X), p(self.DY),
+                                                               *g, p(self.WSW), self.ws_w, s))]
+        return ks
+
+    def outputs(self):
+        return [self.Y, self.DX, self.DW]
+
+
+PASS_OF = {"pack_fwd": "fwd", "conv_fwd": "fwd", "pack_dgrad": "dgrad", "conv_dgrad": "dgrad",
+           "wgrad_partial": "wgrad", "wgrad_reduce": "wgrad", "wgrad": "wgrad"}
+
+
+def time_step(af, ev, stream, step, steps, warmup):
+    ks = step.kernels()
     for _ in range(warmup):
-        run_step(af, x, w, dy, hold, precision, out_type)
+        for _, k in ks:
+            assert k() == 0
     af.sync()
-    # per-pass events: e[3*s + p] .. e[3*s + p + 1]
-    marks = ev.create(3 * steps + 1)
+    marks = ev.create(len(ks) * steps + 1)
     t0 = time.perf_counter()
     ev.record(marks[0], stream)
-    for s in range(steps):
-        y = af.convolve2NN(x, w, ST, PD, DL, precision=precision, out_type=out_type)
-        ev.record(marks[3 * s + 1], stream)
-        dx = af.convolve2GradientNN(dy, x, w, y, ST, PD, DL, af.AF_CONV_GRADIENT_DATA,
-                                    precision=precision, out_type=out_type)
-        ev.record(marks[3 * s + 2], stream)
-        dw = af.convolve2GradientNN(dy, x, w, y, ST, PD, DL, af.AF_CONV_GRADIENT_FILTER,
-                                    precision=precision, out_type=out_type)
-        ev.record(marks[3 * s + 3], stream)
-        hold = [y, dx, dw]
+    i = 1
+    for _ in range(steps):
+        for _, k in ks:
+            rc = k()
+            if rc != 0:
+                raise RuntimeError(f"kernel launch failed with {rc}")
+            ev.record(marks[i], stream)
+            i += 1
     af.sync()
     wall = time.perf_counter() - t0
-    per = np.zeros(3)
-    for s in range(steps):
-        for p in range(3):
-            per[p] += ev.ms(marks[3 * s + p], marks[3 * s + p + 1])
-    total_ms = ev.ms(marks[0], marks[-1])
-    return total_ms / steps, per / steps, wall, hold
+    per_kernel = {n: 0.0 for n, _ in ks}
+    i = 0
+    for _ in range(steps):
+        for n, _ in ks:
+            per_kernel[n] += ev.ms(marks[i], marks[i + 1])
+            i += 1
+    per_kernel = {n: v / steps for n, v in per_kernel.items()}
+    total = ev.ms(marks[0], marks[-1]) / steps
+    return total, per_kernel, wall
+
+
+def time_af_api(af, ev, stream, X, W, DY, out_type, steps, warmup=2):
+    """The step through the af_* C API (pool allocation and dispatch on every
+    call), timed with one HIP event pair around all steps."""
+    hold = []
+    for _ in range(warmup):
+        y = af.convolve2NN(X, W, ST, PD, DL, out_type=out_type)
+        hold = [y, af.convolve2GradientNN(DY, X, W, y, ST, PD, DL, af.AF_CONV_GRADIENT_DATA, out_type=out_type),
+                af.convolve2GradientNN(DY, X, W, y, ST, PD, DL, af.AF_CONV_GRADIENT_FILTER, out_type=out_type)]
+    af.sync()
+    e0, e1 = ev.create(2)
+    ev.record(e0, stream)
+    for _ in range(steps):
+        y = af.convolve2NN(X, W, ST, PD, DL, out_type=out_type)
+        hold = [y, af.convolve2GradientNN(DY, X, W, y, ST, PD, DL, af.AF_CONV_GRADIENT_DATA, out_type=out_type),
+                af.convolve2GradientNN(DY, X, W, y, ST, PD, DL, af.AF_CONV_GRADIENT_FILTER, out_type=out_type)]
+    ev.record(e1, stream)
+    ms = ev.ms(e0, e1) / steps
+    del hold
+    return ms
 
 
 def fp64_errors(x, w, dy, outs):
@@ -318,6 +420,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 error check")
     ap.add_argument("--no-spatial", action="store_true", help="skip the spatial secondary lines")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy (cast_kernel<float, float>) 5 times, "
+                         "to scale rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -328,6 +433,17 @@ def main():
 
     af.lib()
     stream = af.stream()
+    if args.copy_calibration:
+        n = 64 * 1024 * 1024  # floats: 256 MiB
+        A = af.constant(1.0, (n,), af.f32)
+        B = af.constant(0.0, (n,), af.f32)
+        for _ in range(5):
+            rc = af.lib().afhip_k_cast(af.f32, ctypes.c_void_p(B.device_ptr()), af.f32,
+                                       ctypes.c_void_p(A.device_ptr()), n, ctypes.c_void_p(stream))
+            assert rc == 0
+        af.sync()
+        print(json.dumps({"copy_calibration_bytes_each_way": 4 * n, "launches": 5}))
+        return
     ev = HipEvents()
     rng = np.random.default_rng(SEED)
     x = rng.standard_normal(XD).astype(np.float32)
@@ -341,42 +457,53 @@ def main():
     flops_pass = nn_flops(XD, WD, yd)
     # BF16 step: bf16 operands and bf16 results (the af convention: the output
     # dtype is the input dtype), FP32 accumulation inside every kernel
-    ms_bf16, per_bf16, wall_bf16, outs = time_path(
-        af, ev, stream, Xb, Wb, DYb, af.PRECISION_DEFAULT, af.bf16, args.steps, args.warmup)
-    ms_f32, per_f32, _, outs32 = time_path(
-        af, ev, stream, X32, W32, DY32, af.PRECISION_DEFAULT, af.f32, max(3, args.steps // 2),
-        max(1, args.warmup // 2))
+    # whole-step throughput: the af_* API, K steps between one event pair
+    af.sync()
+    t0 = time.perf_counter()
+    ms_bf16 = time_af_api(af, ev, stream, Xb, Wb, DYb, af.bf16, args.steps, args.warmup)
+    wall_bf16 = time.perf_counter() - t0
+    ms_f32 = time_af_api(af, ev, stream, X32, W32, DY32, af.f32, max(3, args.steps // 2),
+                         max(1, args.warmup // 2))
+    # per-kernel durations: the same step through the kernel ABI with an event
+    # after every launch (the events add small gaps; the af_* timing above has
+    # none)
+    stepb = NNStep(af, Xb, Wb, DYb, af.bf16, af.bf16, af.PRECISION_DEFAULT)
+    ms_kabi, kern_bf16, _ = time_step(af, ev, stream, stepb, args.steps, args.warmup)
+    step32 = NNStep(af, X32, W32, DY32, af.f32, af.f32, af.PRECISION_DEFAULT)
+    _, kern_f32, _ = time_step(af, ev, stream, step32, max(3, args.steps // 2), max(1, args.warmup // 2))
     tf_bf16 = 3 * flops_pass / (ms_bf16 * 1e-3) / 1e12
     tf_f32 = 3 * flops_pass / (ms_f32 * 1e-3) / 1e12
+    names = ["fwd", "dgrad", "wgrad"]
+    per_bf16 = [sum(v for k, v in kern_bf16.items() if PASS_OF[k] == n) for n in names]
+    per_f32 = [sum(v for k, v in kern_f32.items() if PASS_OF[k] == n) for n in names]
 
     errs = None
     if not args.no_check:
         # the BF16 operands are the RNE roundings of x, w, dy: the error
         # against FP64 includes that input rounding
-        errs = fp64_errors(x, w, dy, outs)
+        errs = fp64_errors(x, w, dy, stepb.outputs())
 
-    # roofline of the dominant pass (largest average time)
-    names = ["fwd", "dgrad", "wgrad"]
-    k = int(np.argmax(per_bf16))
-    # algorithmic bytes: bf16 operands read once, bf16 result written once
+    # roofline of the dominant kernel (largest average duration)
+    kname = max(kern_bf16, key=kern_bf16.get)
     nx, nw, ny = np.prod(XD), np.prod(WD), np.prod(yd)
-    bytes_pass = {"fwd": 2 * (nx + nw + ny),
-                  "dgrad": 2 * (ny + nw + nx),
-                  "wgrad": 2 * (nx + ny + nw)}[names[k]]
-    t_s = per_bf16[k] * 1e-3
-    t_mem = bytes_pass / (PEAK["hbm_GBs"] * 1e9)
-    t_cmp = flops_pass / (PEAK["bf16_TFs"] * 1e12)
+    kbytes = {"conv_fwd": 2 * (nx + ny) + 2 * nw, "conv_dgrad": 2 * (ny + nx) + 2 * nw,
+              "wgrad_partial": 2 * (nx + ny), "wgrad_reduce": 0, "pack_fwd": 0, "pack_dgrad": 0}
+    bytes_k = float(kbytes.get(kname, 2 * (nx + ny + nw)))
+    flops_k = flops_pass if kname in ("conv_fwd", "conv_dgrad", "wgrad_partial", "wgrad") else 0.0
+    t_s = kern_bf16[kname] * 1e-3
+    t_mem = bytes_k / (PEAK["hbm_GBs"] * 1e9)
+    t_cmp = flops_k / (PEAK["bf16_TFs"] * 1e12)
     if t_mem >= t_cmp:
-        roof = {"bound": "hbm", "achieved": bytes_pass / t_s / 1e9, "peak": PEAK["hbm_GBs"],
-                "unit": "GB/s"}
+        roof = {"bound": "hbm", "achieved": bytes_k / t_s / 1e9, "peak": PEAK["hbm_GBs"], "unit": "GB/s"}
     else:
-        roof = {"bound": "mfma", "achieved": flops_pass / t_s / 1e12, "peak": PEAK["bf16_TFs"],
+        roof = {"bound": "mfma", "achieved": flops_k / t_s / 1e12, "peak": PEAK["bf16_TFs"],
                 "unit": "TFLOP/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
-    roof["kernel"] = f"convolve2_nn {names[k]} (BF16 MFMA implicit GEMM)"
-    roof["algorithmic_bytes"] = int(bytes_pass)
-    roof["algorithmic_flops"] = flops_pass
+    roof["traffic"] = TRAFFIC.get(kname)
+    roof["kernel"] = KERNEL_SYMBOL.get(kname, kname)
+    roof["kernel_avg_us"] = round(kern_bf16[kname] * 1e3, 2)
+    roof["algorithmic_bytes"] = int(bytes_k)
+    roof["algorithmic_flops"] = flops_k
 
     out = {
         "metric": "convolve2_nn fwd+bwd TFLOP/s, BF16-MFMA vs FP32 plain, at rel-L2 <=1e-2",
@@ -399,6 +526,9 @@ def main():
         "bf16_over_fp32": round(tf_bf16 / tf_f32, 3),
         "ms_per_pass_bf16": {n: round(float(v), 4) for n, v in zip(names, per_bf16)},
         "ms_per_pass_fp32": {n: round(float(v), 4) for n, v in zip(names, per_f32)},
+        "us_per_kernel_bf16": {n: round(v * 1e3, 2) for n, v in kern_bf16.items()},
+        "us_per_kernel_fp32": {n: round(v * 1e3, 2) for n, v in kern_f32.items()},
+        "ms_per_step_bf16_kernel_abi_with_events": round(ms_kabi, 4),
         "host_wall_ms_per_step_bf16": round(wall_bf16 / args.steps * 1e3, 4),
         "rel_l2_vs_fp64_bf16": None if errs is None else {n: float(f"{e:.3e}") for n, e in
                                                          zip(names, errs)},

@@ -179,3 +179,41 @@ AFHIP_KAPI int afhip_k_ragged_tile(int variant, int *bx, int *by);
 AFHIP_KAPI int afhip_k_convolve2_ragged(int variant, const void *problems,
                                         const void *items, long long nitems,
                                         int grid, void *stream);
+
+/* The two launches of afhip_k_conv2_nn_patch, separately: _pack writes the
+ * packed weights into the workspace, _run convolves with them (the workspace
+ * may be reused across _run calls while the weights are unchanged). */
+AFHIP_KAPI int afhip_k_conv2_nn_patch_pack(int pass, int in_type, int precision,
+                                           const void *w, const long long *xdims,
+                                           const long long *wdims,
+                                           const long long *ydims,
+                                           const long long *stride,
+                                           const long long *pad,
+                                           const long long *dil, void *workspace,
+                                           long long workspace_bytes,
+                                           void *stream);
+AFHIP_KAPI int afhip_k_conv2_nn_patch_run(int pass, int in_type, int out_type,
+                                          int precision, void *out,
+                                          const void *in,
+                                          const long long *xdims,
+                                          const long long *wdims,
+                                          const long long *ydims,
+                                          const long long *stride,
+                                          const long long *pad,
+                                          const long long *dil,
+                                          const void *workspace,
+                                          long long workspace_bytes, void *stream);
+
+/* The two launches of afhip_k_conv2_nn_wgrad_rows, separately: _partial
+ * writes the FP32 partial tiles into the workspace, _reduce sums them into
+ * dw in a fixed order. */
+AFHIP_KAPI int afhip_k_conv2_nn_wgrad_rows_partial(
+    int in_type, int precision, const void *x, const void *dy,
+    const long long *xdims, const long long *wdims, const long long *ydims,
+    const long long *stride, const long long *pad, const long long *dil,
+    void *workspace, long long workspace_bytes, void *stream);
+AFHIP_KAPI int afhip_k_conv2_nn_wgrad_rows_reduce(
+    int in_type, int out_type, int precision, void *dw, const long long *xdims,
+    const long long *wdims, const long long *ydims, const long long *stride,
+    const long long *pad, const long long *dil, const void *workspace,
+    long long workspace_bytes, void *stream);
