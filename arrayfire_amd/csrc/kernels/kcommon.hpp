@@ -46,7 +46,12 @@ typedef cplx<float> cfloat;
 typedef cplx<double> cdouble;
 
 // bfloat16 / binary16 conversions (round to nearest even; NaN stays NaN).
+// On the device this is gfx950's v_cvt_pk_bf16_f32 (same rounding); the host
+// form is the reference for it.
 __host__ __device__ inline uint16_t f32_to_bf16_bits(float f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_bit_cast(uint16_t, (__bf16)f);
+#endif
     uint32_t u = __builtin_bit_cast(uint32_t, f);
     if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
     uint32_t lsb = (u >> 16) & 1u;

@@ -14,6 +14,10 @@
 // Per (chunk, tap) step the 128 x KC weight tile comes from a pre-packed
 // buffer (one 16-byte load per lane, prefetched a step ahead, double-buffered
 // in LDS); B fragments are read straight from the patch at the tap's offset.
+#include <cstdlib>
+#include <type_traits>
+#include <utility>
+
 #include "kcommon.hpp"
 #include "../../../include/afhip_kernels.h"
 
@@ -48,6 +52,10 @@ struct PatchParams {
     int tiles_per_img;   // ceil(OH*OW / BN)
     int ktiles;          // ceil(K / BM)
     long long ntiles;    // ktiles * tiles_per_img * N
+    // pipelined kernel only
+    int ngroups;         // ceil(RS / Q_TPS)
+    int nseg;            // 16-pixel blocks per input row
+    int items;           // 64 * nseg * PRmax 16-byte patch loads per stage
 };
 
 __device__ inline uint16_t pf32_to_f16_bits(float v) {
@@ -80,41 +88,51 @@ __global__ __launch_bounds__(256) void pack_weights(void *__restrict__ out,
                                                     const TIn *__restrict__ w, int M, int CH,
                                                     int fw, int fh, int wC, int nchunks,
                                                     int ktiles, int dgrad) {
+    // one lane per 16-byte output vector: VPK channels of one packed row
+    // (ktile, chunk, tap, m); consecutive lanes write consecutive vectors
     constexpr int KC = PCfg<MODE>::KC;
+    constexpr int VPK = MODE == PM_F32 ? 4 : 8;
+    constexpr int NV = KC / VPK;  // vectors per row
     const int RS = fw * fh;
-    const long long plane = (long long)ktiles * nchunks * RS * P_BM * KC;
-    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < plane;
-         e += (long long)gridDim.x * blockDim.x) {
-        long long r = e;
-        const int c = (int)(r % KC);
-        r /= KC;
-        const int m = (int)(r % P_BM);
-        r /= P_BM;
-        const int t = (int)(r % RS);
+    const int rows = ktiles * nchunks * RS * P_BM;
+    const long long plane = (long long)rows * KC;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < rows * NV; e += gridDim.x * blockDim.x) {
+        const int row = e / NV, vq = e - (e / NV) * NV;
+        const int m = row % P_BM;
+        int r = row / P_BM;
+        const int t = r % RS;
         r /= RS;
-        const int cc = (int)(r % nchunks);
-        const int kt = (int)(r / nchunks);
-        const int mm = kt * P_BM + m, ch = cc * KC + c;
+        const int cc = r % nchunks, kt = r / nchunks;
+        const int mm = kt * P_BM + m;
         const int i = t % fw, j = t / fw;
-        float v = 0.f;
-        if (mm < M && ch < CH) {
-            long long idx;
-            if (!dgrad)  // w_af[fw-1-i, fh-1-j, ch, mm], KCRS memory (mm*C + ch)
-                idx = (((long long)mm * wC + ch) * fh + (fh - 1 - j)) * fw + (fw - 1 - i);
-            else         // w_af[i, j, mm, ch] with mm the channel, ch the filter
-                idx = (((long long)ch * wC + mm) * fh + j) * fw + i;
-            v = pld(w, idx);
+        // fwd: w_af[fw-1-i, fh-1-j, ch, mm] (KCRS memory, channel stride RS);
+        // dgrad: w_af[i, j, mm, ch] with mm the channel (filter stride wC*RS)
+        const long long base = !dgrad ? (long long)mm * wC * RS + (fh - 1 - j) * fw + (fw - 1 - i)
+                                      : (long long)mm * RS + j * fw + i;
+        const long long step = !dgrad ? RS : (long long)wC * RS;
+        float v[VPK];
+#pragma unroll
+        for (int c = 0; c < VPK; ++c) {
+            const int ch = cc * KC + vq * VPK + c;
+            v[c] = (mm < M && ch < CH) ? pld(w, base + ch * step) : 0.f;
         }
+        const long long e0 = (long long)row * KC + vq * VPK;
         if constexpr (MODE == PM_F32) {
-            reinterpret_cast<float *>(out)[e] = v;
-        } else if constexpr (MODE == PM_BF16) {
-            reinterpret_cast<uint16_t *>(out)[e] = f32_to_bf16_bits(v);
-        } else if constexpr (MODE == PM_F16) {
-            reinterpret_cast<uint16_t *>(out)[e] = pf32_to_f16_bits(v);
+            *reinterpret_cast<float4 *>(reinterpret_cast<float *>(out) + e0) = make_float4(v[0], v[1], v[2], v[3]);
         } else {
-            const uint16_t hb = f32_to_bf16_bits(v);
-            reinterpret_cast<uint16_t *>(out)[e] = hb;
-            reinterpret_cast<uint16_t *>(out)[plane + e] = f32_to_bf16_bits(v - bf16_bits_to_f32(hb));
+            uint16_t h[8], l[8];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                h[c] = MODE == PM_F16 ? pf32_to_f16_bits(v[c]) : f32_to_bf16_bits(v[c]);
+                l[c] = MODE == PM_SPLIT ? f32_to_bf16_bits(v[c] - bf16_bits_to_f32(h[c])) : 0;
+            }
+            *reinterpret_cast<uint4 *>(reinterpret_cast<uint16_t *>(out) + e0) =
+                make_uint4(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16),
+                           h[4] | ((uint32_t)h[5] << 16), h[6] | ((uint32_t)h[7] << 16));
+            if constexpr (MODE == PM_SPLIT)
+                *reinterpret_cast<uint4 *>(reinterpret_cast<uint16_t *>(out) + plane + e0) =
+                    make_uint4(l[0] | ((uint32_t)l[1] << 16), l[2] | ((uint32_t)l[3] << 16),
+                               l[4] | ((uint32_t)l[5] << 16), l[6] | ((uint32_t)l[7] << 16));
         }
     }
 }
@@ -379,6 +397,382 @@ __global__ __launch_bounds__(P_THREADS, 4) void conv_patch(TOut *__restrict__ y,
     }
 }
 
+// ---------------------------------------------------------------- pipelined kernel
+// Same GEMM and packed weights as conv_patch, organised as a software
+// pipeline for one 512-lane workgroup per CU:
+//  * persistent: workgroup b walks tiles b', b' + G, ... (b' = XCD-contiguous
+//    renumbering of b), so neighbouring tiles share halo rows in one L2;
+//  * the patch of the next (tile, channel chunk) stage is read from HBM into
+//    registers with 16-byte loads while the current stage runs its MFMAs, and
+//    written to the other of two LDS patch buffers when the stage ends;
+//  * weights are staged TPS taps at a time (one barrier per group instead of
+//    one per tap), loaded two groups ahead;
+//  * the MFMA operands are (patch, weights), so each lane's accumulators are
+//    4 consecutive output pixels of one filter row: the epilogue stores them
+//    directly (no LDS, no barrier) while the next tile's first patch and
+//    weight groups are already in flight.
+// LDS layout (patch and weights alike): 64 bytes per pixel (or filter row) =
+// 4 groups of 16 bytes (8 bf16/f16 or 4 f32 channels); group g of pixel p sits
+// at p*64 + ((g ^ ((p >> 1) & 3)) << 4). With that swizzle the MFMA fragment
+// reads (16 consecutive pixels x 4 groups per ds_read_b128) run at the full
+// LDS rate on gfx950 for any starting pixel; the patch row pitch is padded to
+// OW mod 16 so that a fragment that wraps to the next output row still reads
+// 16 positions congruent mod 16 to consecutive ones.
+// Gather: each lane loads one 16-byte segment (SEG pixels of one channel) of
+// NCHW; groups of SEG lanes hold SEG channels of the same segment and
+// transpose it in registers (DPP / ds_swizzle exchanges), after which every
+// lane writes one 16-byte channel group of one pixel.
+// WM = 2: 128 filters x 256 pixels (8 waves as 2 x 4 of 64 x 64);
+// WM = 1: 64 filters x 512 pixels (1 x 8), for GEMMs with at most 64 rows
+// (the data gradient of a layer with 64 input channels).
+// Requires W % SEG == 0 and a 16-byte aligned input, and operands stored in
+// their input type (no conversion in the gather).
+constexpr int Q_TPS = 3;    // max taps per weight-staging group (TPS = 3 when taps % 3 == 0, else 1)
+constexpr int Q_THREADS = 512;
+constexpr int Q_PB = 64;    // LDS bytes per pixel / filter row
+
+template <int MODE, int WM>
+struct QCfg {
+    static constexpr int BM = 64 * WM;
+    static constexpr int WN = 8 / WM;
+    static constexpr int BN = 64 * WN;
+    static constexpr int A_BUF = Q_TPS * BM * Q_PB;                               // bytes per weight buffer
+    static constexpr int ASLOT = (Q_TPS * BM * 4 + Q_THREADS - 1) / Q_THREADS;  // 16-B loads per lane
+    static constexpr int NSLOT = WM == 2 ? 4 : 8;  // 16-B patch loads per lane per stage (max)
+};
+
+__device__ inline int q_swz(int p, int g) { return (p << 6) + ((g ^ ((p >> 1) & 3)) << 4); }
+
+// lane exchanges within groups of 2, 4 and 8 lanes
+__device__ inline uint32_t q_x1(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, true);  // quad_perm [1,0,3,2]
+}
+__device__ inline uint32_t q_x2(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, true);  // quad_perm [2,3,0,1]
+}
+__device__ inline uint32_t q_x4(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x101F);  // lane ^ 4 (bit-mask mode)
+}
+
+// 8 x 8 transpose of 16-bit values over lanes l = 0..7 of each group: lane l
+// holds row l (8 values, value e in half e%2 of dword e/2); afterwards lane l
+// holds column l. Must run with all lanes active.
+__device__ inline void q_tr8x8_b16(uint4 &v, int lane) {
+    const bool b0 = lane & 1, b1 = lane & 2, b2 = lane & 4;
+    const uint32_t d0 = v.x, d1 = v.y, d2 = v.z, d3 = v.w;
+    // columns 4-7 of rows 0-3 trade places with columns 0-3 of rows 4-7
+    const uint32_t r0 = q_x4(b2 ? d0 : d2), r1 = q_x4(b2 ? d1 : d3);
+    const uint32_t A0 = b2 ? r0 : d0, A1 = b2 ? r1 : d1, B0 = b2 ? d2 : r0, B1 = b2 ? d3 : r1;
+    // then column pairs within each half
+    const uint32_t u0 = q_x2(b1 ? A0 : A1), u1 = q_x2(b1 ? B0 : B1);
+    const uint32_t V0 = b1 ? A1 : A0, V1 = b1 ? B1 : B0;
+    // then single columns (16-bit halves)
+    const uint32_t selS = b0 ? 0x05040100u : 0x07060302u, selK = b0 ? 0x07060302u : 0x05040100u;
+    const uint32_t R0 = q_x1(__builtin_amdgcn_perm(V1, V0, selS)), R1 = q_x1(__builtin_amdgcn_perm(u1, u0, selS));
+    const uint32_t K0 = __builtin_amdgcn_perm(V1, V0, selK), K1 = __builtin_amdgcn_perm(u1, u0, selK);
+    const uint32_t P = b1 ? K1 : K0, Q = b1 ? R1 : R0, P2 = b1 ? K0 : K1, Q2 = b1 ? R0 : R1;
+    const uint32_t x = b0 ? Q : P, y = b0 ? P : Q, x2 = b0 ? Q2 : P2, y2 = b0 ? P2 : Q2;
+    v.x = __builtin_amdgcn_perm(y, x, 0x05040100u);
+    v.y = __builtin_amdgcn_perm(y2, x2, 0x05040100u);
+    v.z = __builtin_amdgcn_perm(y, x, 0x07060302u);
+    v.w = __builtin_amdgcn_perm(y2, x2, 0x07060302u);
+}
+
+// 4 x 4 transpose of 32-bit values over lanes l = 0..3 of each group.
+__device__ inline void q_tr4x4_b32(uint4 &v, int lane) {
+    const bool b0 = lane & 1, b1 = lane & 2;
+    const uint32_t d0 = v.x, d1 = v.y, d2 = v.z, d3 = v.w;
+    const uint32_t r0 = q_x2(b1 ? d0 : d2), r1 = q_x2(b1 ? d1 : d3);
+    const uint32_t a0 = b1 ? r0 : d0, a1 = b1 ? r1 : d1, c0 = b1 ? d2 : r0, c1 = b1 ? d3 : r1;
+    const uint32_t u0 = q_x1(b0 ? a0 : a1), u1 = q_x1(b0 ? c0 : c1);
+    const uint32_t k0 = b0 ? a1 : a0, k1 = b0 ? c1 : c0;
+    v = b0 ? make_uint4(u0, k0, u1, k1) : make_uint4(k0, u0, k1, u1);
+}
+
+struct QTile {
+    int id, n, kt, q0, q1, oh_a, ih0;
+};
+
+template <int MODE, typename TIn, typename TOut, int WM, int TPS, int PASS>
+__global__ __launch_bounds__(Q_THREADS, 2) void conv_pipe(TOut *__restrict__ y,
+                                                         const TIn *__restrict__ x,
+                                                         const char *__restrict__ wpk,
+                                                         PatchParams p, int vec_out) {
+    using Cfg = PCfg<MODE>;
+    using Q = QCfg<MODE, WM>;
+    constexpr int KC = Cfg::KC;
+    constexpr int ESZ = Cfg::ESZ;
+    static_assert(sizeof(TIn) == ESZ, "the gather stores operands in their input type");
+    constexpr int BM = Q::BM, WN = Q::WN, BN = Q::BN;
+    constexpr int SEG = 16 / ESZ;                 // pixels per 16-byte segment = channels per group
+    constexpr int STEP = P_BM * KC * ESZ;         // packed bytes per (chunk, tap): 128 rows
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    char *lA = smem;                  // [2][Q_TPS][BM][64]
+    char *lP = smem + 2 * Q::A_BUF;   // [2][PRmax][PW][64]   (PW = padded pitch)
+    const int patch_bytes = p.PRmax * p.PW * Q_PB;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave / WN, wn = wave % WN;
+    const int g = lane >> 4;  // 16-byte channel group this lane reads / writes
+    const int P = p.OH * p.OW;
+    const long long HW = (long long)p.H * p.W;
+    const int G = gridDim.x;
+    const int lb = (int)xcd_remap(blockIdx.x, G);
+    const int ntiles = (int)p.ntiles;    // < 2^31 (checked on the host)
+    const int nrest = p.nseg * p.PRmax;  // (16-pixel block, row) pairs per stage; nseg = blocks per row
+
+    // zero both patch buffers once: columns outside the gathered blocks are
+    // never written again
+    for (int i = tid; i < 2 * patch_bytes / 16; i += Q_THREADS)
+        reinterpret_cast<uint4 *>(lP)[i] = make_uint4(0, 0, 0, 0);
+
+    auto decode = [&](int id) {
+        QTile t;
+        t.id = id;
+        const int pt = id % p.tiles_per_img, r = id / p.tiles_per_img;
+        t.n = r % p.N;
+        t.kt = r / p.N;
+        t.q0 = pt * BN;
+        t.q1 = min(t.q0 + BN, P) - 1;
+        t.oh_a = t.q0 / p.OW;
+        t.ih0 = t.oh_a * p.s1 - p.p1;
+        return t;
+    };
+
+    // slot s of this lane: 16-pixel block / row pair rest = s*8 + wave
+    // (wave-uniform), channel g*SEG + lane%SEG, segment (lane%16)/SEG of the block
+    uint4 gx[Q::NSLOT];
+    const int ch_l = g * SEG + (lane & (SEG - 1));
+    const int px_l = (lane & 15) & ~(SEG - 1);
+    auto issue_gather = [&](const QTile &t, int cc) {
+        const TIn *xn = x + (long long)t.n * p.C * HW;
+        const int c = cc * KC + ch_l;
+#pragma unroll
+        for (int s = 0; s < Q::NSLOT; ++s) {
+            gx[s] = make_uint4(0, 0, 0, 0);
+            const int rest = s * 8 + wave;
+            if (rest < nrest) {
+                const int row = rest / p.nseg, blk = rest - (rest / p.nseg) * p.nseg;
+                const int ih = t.ih0 + row, iw = blk * 16 + px_l;
+                if (ih >= 0 && ih < p.H && c < p.C && iw < p.W)
+                    gx[s] = *reinterpret_cast<const uint4 *>(xn + (long long)c * HW + (long long)ih * p.W + iw);
+            }
+        }
+    };
+    auto store_gather = [&](int buf) {
+        char *dstb = lP + buf * patch_bytes;
+#pragma unroll
+        for (int s = 0; s < Q::NSLOT; ++s) {
+            const int rest = s * 8 + wave;
+            if (rest < nrest) {  // wave-uniform: the exchanges below see every lane
+                uint4 v = gx[s];
+                if constexpr (ESZ == 2)
+                    q_tr8x8_b16(v, lane);
+                else
+                    q_tr4x4_b32(v, lane);
+                const int row = rest / p.nseg, blk = rest - (rest / p.nseg) * p.nseg;
+                const int col = p.p0 + blk * 16 + (lane & 15);
+                if (col >= 0) *reinterpret_cast<uint4 *>(dstb + q_swz(row * p.PW + col, g)) = v;
+            }
+        }
+    };
+
+    // Weight groups are loaded two groups ahead into two register sets: set
+    // k & 1 holds group k (the k-th group of this workgroup's sequence of
+    // tiles lb, lb + G, ..., each tile nchunks * ngroups groups). A cursor
+    // (tile step, chunk, group) walks two groups ahead of the compute.
+    static_assert(Q::ASLOT <= 3, "weight slots");
+    uint4 wa0 = make_uint4(0, 0, 0, 0), wa1 = wa0, wa2 = wa0, wb0 = wa0, wb1 = wa0, wb2 = wa0;
+    int at = 0, acc_c = 0, ag = 0;        // cursor: tile step, chunk, group
+    int akt = lb < ntiles ? (lb / p.tiles_per_img) / p.N : 0;
+    auto advance = [&]() {
+        if (++ag == p.ngroups) {
+            ag = 0;
+            if (++acc_c == p.nchunks) {
+                acc_c = 0;
+                ++at;
+                const int id = lb + at * G;
+                akt = id < ntiles ? (id / p.tiles_per_img) / p.N : 0;
+            }
+        }
+    };
+    auto issue_A = [&](auto set) {  // loads the cursor's group, then advances it
+        constexpr int S = decltype(set)::value;
+        uint4 &r0 = S ? wb0 : wa0;
+        uint4 &r1 = S ? wb1 : wa1;
+        uint4 &r2 = S ? wb2 : wa2;
+        if (lb + at * G < ntiles) {
+            const char *src = wpk + ((long long)(akt * p.nchunks + acc_c) * p.RS + ag * TPS) * STEP;
+            constexpr int lim = TPS * BM * 4;
+            auto ld = [&](int s) {
+                const int a = s * Q_THREADS + tid;
+                const int tl = a / (BM * 4), r = (a >> 2) % BM, q = a & 3;
+                return a < lim ? *reinterpret_cast<const uint4 *>(src + tl * STEP + r * (KC * ESZ) + q * 16)
+                               : make_uint4(0, 0, 0, 0);
+            };
+            r0 = ld(0);
+            if constexpr (Q::ASLOT > 1) r1 = ld(1);
+            if constexpr (Q::ASLOT > 2) r2 = ld(2);
+        }
+        advance();
+    };
+    auto store_A = [&](auto set, int buf) {
+        constexpr int S = decltype(set)::value;
+        constexpr int lim = TPS * BM * 4;
+        auto st = [&](int s, const uint4 &v) {
+            const int a = s * Q_THREADS + tid;
+            const int tl = a / (BM * 4), r = (a >> 2) % BM, q = a & 3;
+            if (a < lim) *reinterpret_cast<uint4 *>(lA + buf * Q::A_BUF + tl * BM * Q_PB + q_swz(r, q)) = v;
+        };
+        st(0, S ? wb0 : wa0);
+        if constexpr (Q::ASLOT > 1) st(1, S ? wb1 : wa1);
+        if constexpr (Q::ASLOT > 2) st(2, S ? wb2 : wa2);
+    };
+    using Set0 = std::integral_constant<int, 0>;
+    using Set1 = std::integral_constant<int, 1>;
+
+    // Accumulators: the MFMA's A operand is the patch (16 pixels x 32
+    // channels) and B the weights, so acc[i][j][v] = out[pixel][filter] with
+    // pixel = q0 + wn*64 + j*16 + 4*(lane/16) + v and filter = kt*BM + wm*64 +
+    // i*16 + lane%16: each lane owns 4 consecutive pixels of one filter row.
+    pf32x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = pf32x4{0.f, 0.f, 0.f, 0.f};
+
+    // weight fragment rows wm*64 + i*16 + lane%16: the swizzle key depends on lane%16 only
+    const int abase = q_swz(wm * 64 + (lane & 15), g);
+    int k = 0;     // weight groups done (buffer parity)
+    int sbuf = 0;  // patch buffer of the current stage
+    QTile cur = decode(lb);
+    if (cur.id < ntiles) {
+        issue_A(Set0{});
+        issue_A(Set1{});
+        issue_gather(cur, 0);
+    }
+    __syncthreads();  // zeroed patch buffers visible
+    while (cur.id < ntiles) {
+        const QTile nxt = decode(cur.id + G);
+        const bool has_next = nxt.id < ntiles;
+        int bpos[4];  // patch position of this lane's pixel in fragment j (tap 0)
+#pragma unroll
+        for (int ni = 0; ni < 4; ++ni) {
+            int q = cur.q0 + wn * 64 + ni * 16 + (lane & 15);
+            if (q > cur.q1) q = cur.q1;  // dead pixels read a live one; never stored
+            const int oh = q / p.OW, ow = q - oh * p.OW;
+            bpos[ni] = ((oh - cur.oh_a) * p.s1) * p.PW + ow * p.s0;
+        }
+        for (int cc = 0; cc < p.nchunks; ++cc) {
+            store_gather(sbuf);
+            const char *lp = lP + sbuf * patch_bytes;
+            for (int gq = 0; gq < p.ngroups; ++gq) {
+                // publish this group's weights (and, at gq == 0, the stage's
+                // patch), then refill the freed register set with group k + 2
+                if (k & 1) {
+                    store_A(Set1{}, 1);
+                    __syncthreads();
+                    issue_A(Set1{});
+                } else {
+                    store_A(Set0{}, 0);
+                    __syncthreads();
+                    issue_A(Set0{});
+                }
+                // next stage's patch: in flight for the whole stage
+                if (gq == 0) {
+                    const bool chunk = cc + 1 < p.nchunks;
+                    if (chunk || has_next) issue_gather(chunk ? cur : nxt, chunk ? cc + 1 : 0);
+                }
+                const char *aw = lA + (k & 1) * Q::A_BUF;
+                // TPS taps, straight-line: fragments of tap u + 1 are read
+                // from LDS while the MFMAs of tap u run
+                using FragT = typename std::conditional<MODE == PM_F32, float4, pu16x8>::type;
+                FragT af[2][4], bfr[2][4];
+                auto load_frags = [&](int u, FragT (&fa)[4], FragT (&fb)[4]) {
+                    const int tap = gq * TPS + u;
+                    const int ti = tap % p.fw, tj = tap / p.fw;
+                    const int toff = tj * p.d1 * p.PW + ti * p.d0;
+                    const char *au = aw + u * BM * Q_PB + abase;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) fa[i] = *reinterpret_cast<const FragT *>(au + i * 16 * Q_PB);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) fb[j] = *reinterpret_cast<const FragT *>(lp + q_swz(bpos[j] + toff, g));
+                };
+                load_frags(0, af[0], bfr[0]);
+#pragma unroll
+                for (int u = 0; u < TPS; ++u) {
+                    if (u + 1 < TPS) load_frags(u + 1, af[(u + 1) & 1], bfr[(u + 1) & 1]);
+                    const FragT(&fa)[4] = af[u & 1];
+                    const FragT(&fb)[4] = bfr[u & 1];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            if constexpr (MODE == PM_F32) {
+                                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[j].x, fa[i].x, acc[i][j], 0, 0, 0);
+                                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[j].y, fa[i].y, acc[i][j], 0, 0, 0);
+                                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[j].z, fa[i].z, acc[i][j], 0, 0, 0);
+                                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[j].w, fa[i].w, acc[i][j], 0, 0, 0);
+                            } else if constexpr (MODE == PM_BF16) {
+                                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                    __builtin_bit_cast(pbf16x8, fb[j]), __builtin_bit_cast(pbf16x8, fa[i]), acc[i][j], 0, 0, 0);
+                            } else {
+                                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                                    __builtin_bit_cast(pf16x8, fb[j]), __builtin_bit_cast(pf16x8, fa[i]), acc[i][j], 0, 0, 0);
+                            }
+                        }
+                }
+                ++k;
+            }
+            sbuf ^= 1;
+        }
+
+        // ---- epilogue: straight from the accumulators, 4 consecutive pixels
+        // of one filter row per lane and fragment (no LDS, no barrier; the
+        // next tile's first patch and weight groups are already in flight)
+        TOut *yn = y + (long long)cur.n * p.K * (long long)P;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int kf = cur.kt * BM + wm * 64 + i * 16 + (lane & 15);
+            if (kf >= p.K) continue;
+            TOut *yrow = yn + (long long)kf * P;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int q = cur.q0 + wn * 64 + j * 16 + 4 * (lane >> 4);
+                if (q > cur.q1) continue;
+                const pf32x4 v = acc[i][j];
+                if constexpr (sizeof(TOut) == 4) {
+                    if (vec_out && q + 3 <= cur.q1) {
+                        *reinterpret_cast<float4 *>(yrow + q) = make_float4(v[0], v[1], v[2], v[3]);
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < 4; ++e)
+                            if (q + e <= cur.q1) yrow[q + e] = v[e];
+                    }
+                } else {
+                    uint16_t hb[4];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) hb[e] = MODE == PM_F16 ? pf32_to_f16_bits(v[e]) : f32_to_bf16_bits(v[e]);
+                    uint16_t *d16 = reinterpret_cast<uint16_t *>(yrow) + q;
+                    if (vec_out && q + 3 <= cur.q1) {
+                        *reinterpret_cast<uint2 *>(d16) =
+                            make_uint2((uint32_t)hb[0] | ((uint32_t)hb[1] << 16), (uint32_t)hb[2] | ((uint32_t)hb[3] << 16));
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < 4; ++e)
+                            if (q + e <= cur.q1) d16[e] = hb[e];
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = pf32x4{0.f, 0.f, 0.f, 0.f};
+        cur = nxt;
+    }
+}
+
 template <int MODE>
 static size_t lds_bytes(const PatchParams &p) {
     using Cfg = PCfg<MODE>;
@@ -407,6 +801,92 @@ static bool plan(PatchParams &p) {
     return lds_bytes<MODE>(p) <= kPatchLdsLimit && p.PW > 0 && p.PRmax > 0;
 }
 
+// Geometry of the pipelined kernel (q, from the conv_patch plan p; q.PW is
+// the padded patch row pitch). Returns false when the operands are converted
+// on the way in, the input is not a whole number of aligned 16-byte segments
+// per row, a stage's patch needs more than NSLOT loads per lane, or the two
+// patch buffers and two weight buffers do not fit in LDS. Setting
+// AFHIP_NN_PIPE=0 in the environment disables the pipelined kernel.
+template <int MODE, typename TIn>
+static bool plan_pipe(PatchParams &q, const PatchParams &p, const void *x, int &wm) {
+    if constexpr (MODE == PM_SPLIT || sizeof(TIn) != PCfg<MODE>::ESZ) {
+        return false;
+    } else {
+        const char *off = std::getenv("AFHIP_NN_PIPE");
+        if (off && off[0] == '0') return false;
+        constexpr int SEG = 16 / (int)sizeof(TIn);
+        q = p;
+        if (q.W % SEG != 0 || (reinterpret_cast<uintptr_t>(x) & 15) != 0) return false;
+        wm = q.K <= 64 ? 1 : 2;
+        const int BM = 64 * wm, BN = 64 * (8 / wm);
+        const int P = q.OH * q.OW;
+        const int span = BN >= P ? q.OH : (min(BN - 1, P - 1) / q.OW + 2);
+        const int rows = min(span, q.OH);
+        q.PRmax = (rows - 1) * q.s1 + (q.fh - 1) * q.d1 + 1;
+        q.tiles_per_img = (P + BN - 1) / BN;
+        q.ktiles = (q.K + BM - 1) / BM;
+        q.ntiles = (long long)q.ktiles * q.tiles_per_img * q.N;
+        q.ngroups = q.RS % Q_TPS == 0 ? q.RS / Q_TPS : q.RS;  // TPS = 3 or 1
+        q.nseg = (q.W + 15) / 16;  // 16-pixel blocks per input row
+        // padded row pitch: holds every gathered block; for stride 1 it is
+        // congruent to OW mod 16 (fragments that wrap rows stay conflict-free)
+        const int need = max(p.PW, q.p0 + q.nseg * 16);
+        q.PW = q.s0 == 1 ? q.OW + 16 * ((need - q.OW + 15) / 16) : need;
+        const int nslot = wm == 2 ? QCfg<MODE, 2>::NSLOT : QCfg<MODE, 1>::NSLOT;
+        if ((long long)q.nseg * q.PRmax > 8LL * nslot || q.ntiles >= (1LL << 31) - (1LL << 20)) return false;
+        q.items = 64 * q.nseg * q.PRmax;
+        const size_t lds = (size_t)2 * Q_TPS * BM * Q_PB + (size_t)2 * q.PRmax * q.PW * Q_PB;
+        return lds <= kPatchLdsLimit;
+    }
+}
+
+static int device_cu_count() {
+    static int cus[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (cus[dev] <= 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cus[dev] = n;
+    }
+    return cus[dev];
+}
+
+template <int MODE, typename TIn, typename TOut>
+static int launch_pipe(void *y, const void *x, const void *ws, const PatchParams &q, int wm, int dgrad,
+                       hipStream_t s) {
+    const int BM = 64 * wm;
+    const size_t lds = (size_t)2 * Q_TPS * BM * Q_PB + (size_t)2 * q.PRmax * q.PW * Q_PB;
+    using Fn = void (*)(TOut *, const TIn *, const char *, PatchParams, int);
+    const bool t3 = q.RS % Q_TPS == 0;
+    Fn fn;
+    if (wm == 1)
+        fn = t3 ? (dgrad ? conv_pipe<MODE, TIn, TOut, 1, 3, 1> : conv_pipe<MODE, TIn, TOut, 1, 3, 0>)
+                : (dgrad ? conv_pipe<MODE, TIn, TOut, 1, 1, 1> : conv_pipe<MODE, TIn, TOut, 1, 1, 0>);
+    else
+        fn = t3 ? (dgrad ? conv_pipe<MODE, TIn, TOut, 2, 3, 1> : conv_pipe<MODE, TIn, TOut, 2, 3, 0>)
+                : (dgrad ? conv_pipe<MODE, TIn, TOut, 2, 1, 1> : conv_pipe<MODE, TIn, TOut, 2, 1, 0>);
+    const int slot = ((wm - 1) * 2 + dgrad) * 2 + (t3 ? 1 : 0);
+    static bool attr_set[8] = {false, false, false, false, false, false, false, false};
+    if (!attr_set[slot]) {
+        if (hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)kPatchLdsLimit) != hipSuccess)
+            return K_ERR_RUNTIME;
+        attr_set[slot] = true;
+    }
+    // persistent grid: as many workgroups as can be resident at once
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn, Q_THREADS, lds) != hipSuccess ||
+        per_cu <= 0)
+        per_cu = 1;
+    long long grid = (long long)device_cu_count() * per_cu;
+    if (grid > q.ntiles) grid = q.ntiles;
+    const int P = q.OH * q.OW;
+    const int vec_out = (P % 4 == 0) && ((reinterpret_cast<uintptr_t>(y) & 15) == 0);
+    fn<<<dim3((unsigned)grid), Q_THREADS, lds, s>>>((TOut *)y, (const TIn *)x, (const char *)ws, q, vec_out);
+    return launch_status();
+}
+
 // what: 1 = pack the weights into ws, 2 = run the convolution with the
 // weights already packed in ws, 3 = both.
 template <int MODE, typename TIn, typename TOut>
@@ -414,16 +894,23 @@ static int launch(void *y, const void *x, const void *w, PatchParams &p, int dgr
                   int what, hipStream_t s) {
     using Cfg = PCfg<MODE>;
     if (what & 1) {
-        const long long plane = (long long)p.ktiles * p.nchunks * p.RS * P_BM * Cfg::KC;
+        const long long rows = (long long)p.ktiles * p.nchunks * p.RS * P_BM;
         // weights: K x C x fh x fw of the original filter; for dgrad the
         // GEMM's M is the original C and its channels are the original K
         const int wC = dgrad ? p.K : p.C;
-        long long blocks = (plane + 255) / 256;
+        const long long vecs = rows * (Cfg::KC / (MODE == PM_F32 ? 4 : 8));
+        if (vecs >= (1LL << 31)) return K_ERR_SIZE;
+        long long blocks = (vecs + 255) / 256;
         if (blocks > kMaxGridBlocks) blocks = kMaxGridBlocks;
         pack_weights<MODE, TIn><<<(unsigned)blocks, 256, 0, s>>>(ws, (const TIn *)w, p.K, p.C, p.fw,
                                                                    p.fh, wC, p.nchunks, p.ktiles, dgrad);
     }
     if (what & 2) {
+        PatchParams q;
+        int wm = 0;
+        if constexpr (MODE != PM_SPLIT && sizeof(TIn) == PCfg<MODE>::ESZ) {
+            if (plan_pipe<MODE, TIn>(q, p, x, wm)) return launch_pipe<MODE, TIn, TOut>(y, x, ws, q, wm, dgrad, s);
+        }
         const size_t lds = lds_bytes<MODE>(p);
         auto fn = dgrad ? conv_patch<MODE, TIn, TOut, 1> : conv_patch<MODE, TIn, TOut, 0>;
         static bool attr_set[2] = {false, false};

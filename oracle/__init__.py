@@ -410,7 +410,7 @@ def conv2_nn_direct_f64(signal, filt, stride=(1, 1), padding=(0, 0),
     for i in range(fd[0]):
         for j in range(fd[1]):
             xs = _tap_slice(xp, i, j, dilation, stride, ow, oh)  # [ow,oh,C,N]
-            y += np.einsum("abcn,ck->abkn", xs, w[fd[0] - 1 - i, fd[1] - 1 - j])
+            y += np.einsum("abcn,ck->abkn", xs, w[fd[0] - 1 - i, fd[1] - 1 - j], optimize=True)
     return y
 
 
@@ -426,7 +426,7 @@ def conv2_data_grad_direct_f64(dy, signal, filt, stride=(1, 1),
                    sd[1] + 2 * padding[1] + stride[1], sd[2], sd[3]))
     for i in range(fd[0]):
         for j in range(fd[1]):
-            t = np.einsum("abkn,ck->abcn", g, w[fd[0] - 1 - i, fd[1] - 1 - j])
+            t = np.einsum("abkn,ck->abcn", g, w[fd[0] - 1 - i, fd[1] - 1 - j], optimize=True)
             a0, a1 = i * dilation[0], j * dilation[1]
             xp[a0:a0 + (ow - 1) * stride[0] + 1:stride[0],
                a1:a1 + (oh - 1) * stride[1] + 1:stride[1]] += t
@@ -445,7 +445,7 @@ def conv2_filter_grad_direct_f64(dy, signal, filt, stride=(1, 1),
     for i in range(fd[0]):
         for j in range(fd[1]):
             xs = _tap_slice(xp, i, j, dilation, stride, ow, oh)
-            dw[fd[0] - 1 - i, fd[1] - 1 - j] = np.einsum("abcn,abkn->ck", xs, g)
+            dw[fd[0] - 1 - i, fd[1] - 1 - j] = np.einsum("abcn,abkn->ck", xs, g, optimize=True)
     return dw
 
 
