@@ -6,11 +6,12 @@
 //
 //  * sep_pass<InT,AccT,DIM>: one pass, one lane per output (the two-launch
 //    form, used for non-f32 types and when the caller asks for the temp).
-//  * sep_fused_f32: both passes in one kernel. A workgroup owns a
-//    64 (dim0) x 64 (dim1) output tile. It stages the (64+cf-1) x (64+rf-1)
-//    input halo in LDS, runs the column pass into a second LDS tile of
-//    64 x (64+rf-1) (rounded to float, the reference's temp precision), then
-//    the row pass to HBM: one read and one write of every element.
+//  * sep_fused_f32<F>: both passes in one kernel, for filters of at most F
+//    taps. A workgroup owns a 64 (dim0) x 64 (dim1) output tile. It stages the
+//    (64+cf-1) x (64+rf-1) input halo in LDS, runs the column pass into a
+//    second LDS tile of 64 x (64+F-1) (rounded to float, the reference's temp
+//    precision), then the row pass to HBM: one read and one write of every
+//    element.
 #include "kcommon.hpp"
 #include "../../../include/afhip_kernels.h"
 
@@ -68,17 +69,26 @@ struct SepFusedParams {
     int cf, rf, sx, sy;
 };
 
-// Tap counts are runtime values (up to SF_MAXF); filters are staged in LDS.
+// Both passes with the tap count bucketed to F >= max(cf, rf): taps k >= cf
+// (k >= rf) are zero and read halo cells that are zeroed (not image data), so
+// they add exactly +0. Taps are uniform: they live in scalar registers.
+//  * column pass: a lane owns 8 consecutive dim0 outputs of one halo row and
+//    slides over a register window of 8 + F - 1 inputs read as float4s;
+//  * row pass: a lane owns one dim0 column and 16 consecutive dim1 outputs,
+//    reading each of the 16 + F - 1 temp values once.
+template <int F>
 __global__ __launch_bounds__(256) void sep_fused_f32(float *__restrict__ out,
                                                      const float *__restrict__ in,
                                                      const float *__restrict__ cfilt,
                                                      const float *__restrict__ rfilt,
                                                      SepFusedParams p) {
     constexpr int B = SF_B;
-    constexpr int LWMAX = B + SF_MAXF - 1;  // 96
-    __shared__ float xin[LWMAX * LWMAX];    // (B+cf-1) x (B+rf-1), pitch LW
-    __shared__ float tmid[B * LWMAX];       // B x (B+rf-1), pitch B
-    __shared__ float wc[SF_MAXF], wr[SF_MAXF];
+    constexpr int L = B + F - 1;                     // halo rows / columns
+    constexpr int TX = 8, TY = 16;
+    constexpr int NW = (TX + F - 1 + 3) / 4;         // float4s per column-pass window
+    constexpr int LP = ((B - TX + 4 * NW) > L ? (B - TX + 4 * NW) : L + 3) & ~3;  // xin pitch
+    __shared__ __attribute__((aligned(16))) float xin[L * LP];
+    __shared__ __attribute__((aligned(16))) float tmid[L * B];
 
     const dim_t blk = xcd_remap(blockIdx.x, p.nblk);
     const dim_t bx = blk % p.nbx;
@@ -89,46 +99,71 @@ __global__ __launch_bounds__(256) void sep_fused_f32(float *__restrict__ out,
     out += b * p.O0 * p.O1;
 
     const int cf = p.cf, rf = p.rf;
-    const int LW = B + cf - 1, LH = B + rf - 1;
-    if (threadIdx.x < cf) wc[threadIdx.x] = cfilt[cf - 1 - threadIdx.x];
-    if (threadIdx.x >= 64 && threadIdx.x - 64 < rf) wr[threadIdx.x - 64] = rfilt[rf - 1 - (threadIdx.x - 64)];
+    float wc[F], wr[F];  // flipped taps, zero beyond the filter length
+#pragma unroll
+    for (int k = 0; k < F; ++k) {
+        wc[k] = k < cf ? cfilt[cf - 1 - k] : 0.f;
+        wr[k] = k < rf ? rfilt[rf - 1 - k] : 0.f;
+    }
 
     const dim_t ox0 = bx * B, oy0 = by * B;
     const dim_t ix0 = ox0 + p.sx - (cf - 1), iy0 = oy0 + p.sy - (rf - 1);
-    load_halo_tile_f32<256, (LWMAX + 6) / 4>(xin, in, p.S0, p.S1, ix0, iy0, LH, LW, LW);
+    const int LWr = B + cf - 1, LHr = B + rf - 1;  // halo the filters read
+    load_halo_tile_f32<256, (L + 6) / 4>(xin, in, p.S0, p.S1, ix0, iy0, LHr, LWr, LP);
+    // zero the cells only the padded taps reach (disjoint from the halo)
+    for (int i = threadIdx.x; i < L * LP; i += 256) {
+        const int r = i / LP, c = i - (i / LP) * LP;
+        if (r >= LHr || c >= LWr) xin[i] = 0.f;
+    }
     __syncthreads();
 
-    // Column pass: tmid(x, r) = sum_k xin(x + k, r) * wc[k] (flipped taps).
-    // Lane owns column x = tid % 64 and rows r = tid/64 + 4q.
+    // Column pass: tmid(x, r) = sum_k xin(x + k, r) * wc[k], r < L.
     {
-        const int x = threadIdx.x & 63;
-        for (int r = threadIdx.x >> 6; r < LH; r += 4) {
-            float acc = 0.f;
-            const float *row = xin + r * LW + x;
-            for (int k = 0; k < cf; ++k) acc = __builtin_fmaf(row[k], wc[k], acc);
-            tmid[r * B + x] = acc;
+        const int x0 = (threadIdx.x & 7) * TX;
+        for (int r = threadIdx.x >> 3; r < L; r += 32) {
+            float win[4 * NW];
+            const float4 *src = reinterpret_cast<const float4 *>(xin + r * LP + x0);
+#pragma unroll
+            for (int q = 0; q < NW; ++q) {
+                const float4 v = src[q];
+                win[4 * q] = v.x;
+                win[4 * q + 1] = v.y;
+                win[4 * q + 2] = v.z;
+                win[4 * q + 3] = v.w;
+            }
+            float acc[TX];
+#pragma unroll
+            for (int e = 0; e < TX; ++e) {
+                float a = 0.f;
+#pragma unroll
+                for (int k = 0; k < F; ++k) a = __builtin_fmaf(win[e + k], wc[k], a);
+                acc[e] = a;
+            }
+            float4 *dst = reinterpret_cast<float4 *>(tmid + r * B + x0);
+            dst[0] = make_float4(acc[0], acc[1], acc[2], acc[3]);
+            dst[1] = make_float4(acc[4], acc[5], acc[6], acc[7]);
         }
     }
     __syncthreads();
 
-    // Row pass: out(x, y) = sum_k tmid(x, y + k) * wr[k]. Lane owns column x
-    // and 16 consecutive rows, sliding down the column.
+    // Row pass: out(x, y) = sum_k tmid(x, y + k) * wr[k].
     {
         const int x = threadIdx.x & 63;
-        const int y0 = (threadIdx.x >> 6) * 16;
-        float acc[16];
+        const int y0 = (threadIdx.x >> 6) * TY;
+        float acc[TY];
 #pragma unroll
-        for (int q = 0; q < 16; ++q) acc[q] = 0.f;
-        for (int k = 0; k < rf; ++k) {
-            const float w = wr[k];
+        for (int q = 0; q < TY; ++q) acc[q] = 0.f;
 #pragma unroll
-            for (int q = 0; q < 16; ++q)
-                acc[q] = __builtin_fmaf(tmid[(y0 + q + k) * B + x], w, acc[q]);
+        for (int q = 0; q < TY + F - 1; ++q) {
+            const float v = tmid[(y0 + q) * B + x];
+#pragma unroll
+            for (int k = 0; k < F; ++k)
+                if (q - k >= 0 && q - k < TY) acc[q - k] = __builtin_fmaf(v, wr[k], acc[q - k]);
         }
         const dim_t gx = ox0 + x;
         if (gx < p.O0) {
 #pragma unroll
-            for (int q = 0; q < 16; ++q) {
+            for (int q = 0; q < TY; ++q) {
                 const dim_t gy = oy0 + y0 + q;
                 if (gy < p.O1) out[gy * p.O0 + gx] = acc[q];
             }
@@ -195,8 +230,22 @@ extern "C" int afhip_k_convolve2_sep(int dtype, void *out, const long long *odim
         f.sy = expand ? 0 : (int)(rflen >> 1);
         if (f.nblk == 0) return K_OK;
         if (f.nblk > 0x7fffffffLL) return K_ERR_SIZE;
-        sep_fused_f32<<<dim3((unsigned)f.nblk), 256, 0, s>>>(
-            (float *)out, (const float *)in, (const float *)cfilt, (const float *)rfilt, f);
+        const int fm = (int)(cflen > rflen ? cflen : rflen);
+#define SEPF(FB)                                                                                  \
+    sep_fused_f32<FB><<<dim3((unsigned)f.nblk), 256, 0, s>>>((float *)out, (const float *)in,     \
+                                                             (const float *)cfilt, (const float *)rfilt, f)
+        if (fm <= 3) SEPF(3);
+        else if (fm <= 5) SEPF(5);
+        else if (fm <= 7) SEPF(7);
+        else if (fm <= 9) SEPF(9);
+        else if (fm <= 11) SEPF(11);
+        else if (fm <= 13) SEPF(13);
+        else if (fm <= 15) SEPF(15);
+        else if (fm <= 17) SEPF(17);
+        else if (fm <= 21) SEPF(21);
+        else if (fm <= 25) SEPF(25);
+        else SEPF(SF_MAXF);
+#undef SEPF
         return launch_status();
     }
     switch (dtype) {
