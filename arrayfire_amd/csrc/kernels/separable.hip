@@ -7,7 +7,8 @@
 //  * sep_pass<InT,AccT,DIM>: one pass, one lane per output (the two-launch
 //    form, used for non-f32 types and when the caller asks for the temp).
 //  * sep_fused_f32<F>: both passes in one kernel, for filters of at most F
-//    taps. A workgroup owns a 64 (dim0) x 64 (dim1) output tile. It stages the
+//    taps (F = 3, 5, ..., 17, 33). A workgroup owns a 64 (dim0) x 64 (dim1)
+//    output tile. It stages the
 //    (64+cf-1) x (64+rf-1) input halo in LDS, runs the column pass into a
 //    second LDS tile of 64 x (64+F-1) (rounded to float, the reference's temp
 //    precision), then the row pass to HBM: one read and one write of every
@@ -242,8 +243,6 @@ extern "C" int afhip_k_convolve2_sep(int dtype, void *out, const long long *odim
         else if (fm <= 13) SEPF(13);
         else if (fm <= 15) SEPF(15);
         else if (fm <= 17) SEPF(17);
-        else if (fm <= 21) SEPF(21);
-        else if (fm <= 25) SEPF(25);
         else SEPF(SF_MAXF);
 #undef SEPF
         return launch_status();
