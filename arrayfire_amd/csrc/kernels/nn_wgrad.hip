@@ -30,7 +30,7 @@ typedef float wf32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t wu32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int WG_BK = 64, WG_BC = 64, WG_THREADS = 256;
-constexpr size_t WG_LDS_LIMIT = 78 * 1024;
+constexpr size_t WG_LDS_LIMIT = 150 * 1024;
 
 struct WGParams {
     int W, H, C, N, OW, OH, K;
@@ -324,8 +324,9 @@ static bool wg_plan(WGParams &p, int fw, int s0, int d0) {
     const int maxe = (p.OWp - 8) * s0 + (p.PADL - p.p0) + 8 * s0 + fw + 4;
     p.PWpad = ((maxe + 7) & ~7) + 8;
     if (p.PADL - p.p0 < 0) return false;
-    // rows per block: about 256 padded pixels, within the LDS budget
-    int r = 256 / p.OWp;
+    // rows per block: up to about 512 padded pixels, within the LDS budget
+    // (one workgroup per CU: taller blocks re-read fewer halo rows of x)
+    int r = 512 / p.OWp;
     if (r < 1) r = 1;
     if (r > p.OH) r = p.OH;
     for (; r >= 1; --r) {

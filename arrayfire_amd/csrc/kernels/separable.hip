@@ -70,14 +70,15 @@ struct SepFusedParams {
     int cf, rf, sx, sy;
 };
 
-// Both passes with the tap count bucketed to F >= max(cf, rf): taps k >= cf
-// (k >= rf) are zero and read halo cells that are zeroed (not image data), so
-// they add exactly +0. Taps are uniform: they live in scalar registers.
+// Both passes with the tap count bucketed to F >= max(cf, rf): the loops are
+// unrolled to F and taps k >= cf (k >= rf) are skipped by uniform branches
+// (never multiplied by 0: the image may hold Inf or NaN); EXACT (cf == rf ==
+// F) has no branches. Taps are uniform: they live in scalar registers.
 //  * column pass: a lane owns 8 consecutive dim0 outputs of one halo row and
 //    slides over a register window of 8 + F - 1 inputs read as float4s;
-//  * row pass: a lane owns one dim0 column and 16 consecutive dim1 outputs,
-//    reading each of the 16 + F - 1 temp values once.
-template <int F>
+//  * row pass: a lane owns one dim0 column and 16 consecutive dim1 outputs;
+//    the 16 + F - 1 temp values it reads are loaded once into registers.
+template <int F, bool EXACT>
 __global__ __launch_bounds__(256) void sep_fused_f32(float *__restrict__ out,
                                                      const float *__restrict__ in,
                                                      const float *__restrict__ cfilt,
@@ -111,7 +112,7 @@ __global__ __launch_bounds__(256) void sep_fused_f32(float *__restrict__ out,
     const dim_t ix0 = ox0 + p.sx - (cf - 1), iy0 = oy0 + p.sy - (rf - 1);
     const int LWr = B + cf - 1, LHr = B + rf - 1;  // halo the filters read
     load_halo_tile_f32<256, (L + 6) / 4>(xin, in, p.S0, p.S1, ix0, iy0, LHr, LWr, LP);
-    // zero the cells only the padded taps reach (disjoint from the halo)
+    // zero the cells outside the halo (read into registers, never used)
     for (int i = threadIdx.x; i < L * LP; i += 256) {
         const int r = i / LP, c = i - (i / LP) * LP;
         if (r >= LHr || c >= LWr) xin[i] = 0.f;
@@ -134,12 +135,12 @@ __global__ __launch_bounds__(256) void sep_fused_f32(float *__restrict__ out,
             }
             float acc[TX];
 #pragma unroll
-            for (int e = 0; e < TX; ++e) {
-                float a = 0.f;
+            for (int e = 0; e < TX; ++e) acc[e] = 0.f;
 #pragma unroll
-                for (int k = 0; k < F; ++k) a = __builtin_fmaf(win[e + k], wc[k], a);
-                acc[e] = a;
-            }
+            for (int k = 0; k < F; ++k)
+                if (EXACT || k < cf)  // uniform: taps past the filter are skipped, not multiplied by 0
+#pragma unroll
+                    for (int e = 0; e < TX; ++e) acc[e] = __builtin_fmaf(win[e + k], wc[k], acc[e]);
             float4 *dst = reinterpret_cast<float4 *>(tmid + r * B + x0);
             dst[0] = make_float4(acc[0], acc[1], acc[2], acc[3]);
             dst[1] = make_float4(acc[4], acc[5], acc[6], acc[7]);
@@ -151,16 +152,17 @@ __global__ __launch_bounds__(256) void sep_fused_f32(float *__restrict__ out,
     {
         const int x = threadIdx.x & 63;
         const int y0 = (threadIdx.x >> 6) * TY;
+        float v[TY + F - 1];  // the column's temp values this lane's outputs read
+#pragma unroll
+        for (int q = 0; q < TY + F - 1; ++q) v[q] = tmid[(y0 + q) * B + x];
         float acc[TY];
 #pragma unroll
         for (int q = 0; q < TY; ++q) acc[q] = 0.f;
 #pragma unroll
-        for (int q = 0; q < TY + F - 1; ++q) {
-            const float v = tmid[(y0 + q) * B + x];
+        for (int k = 0; k < F; ++k)
+            if (EXACT || k < rf)  // uniform, as in the column pass
 #pragma unroll
-            for (int k = 0; k < F; ++k)
-                if (q - k >= 0 && q - k < TY) acc[q - k] = __builtin_fmaf(v, wr[k], acc[q - k]);
-        }
+                for (int q = 0; q < TY; ++q) acc[q] = __builtin_fmaf(v[q + k], wr[k], acc[q]);
         const dim_t gx = ox0 + x;
         if (gx < p.O0) {
 #pragma unroll
@@ -232,9 +234,15 @@ extern "C" int afhip_k_convolve2_sep(int dtype, void *out, const long long *odim
         if (f.nblk == 0) return K_OK;
         if (f.nblk > 0x7fffffffLL) return K_ERR_SIZE;
         const int fm = (int)(cflen > rflen ? cflen : rflen);
-#define SEPF(FB)                                                                                  \
-    sep_fused_f32<FB><<<dim3((unsigned)f.nblk), 256, 0, s>>>((float *)out, (const float *)in,     \
-                                                             (const float *)cfilt, (const float *)rfilt, f)
+#define SEPF(FB)                                                                                           \
+    do {                                                                                                   \
+        if (cflen == FB && rflen == FB)                                                                    \
+            sep_fused_f32<FB, true><<<dim3((unsigned)f.nblk), 256, 0, s>>>(                                \
+                (float *)out, (const float *)in, (const float *)cfilt, (const float *)rfilt, f);           \
+        else                                                                                               \
+            sep_fused_f32<FB, false><<<dim3((unsigned)f.nblk), 256, 0, s>>>(                               \
+                (float *)out, (const float *)in, (const float *)cfilt, (const float *)rfilt, f);           \
+    } while (0)
         if (fm <= 3) SEPF(3);
         else if (fm <= 5) SEPF(5);
         else if (fm <= 7) SEPF(7);

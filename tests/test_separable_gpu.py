@@ -93,10 +93,9 @@ def test_scalar_filters(af, rng):
 
 @pytest.mark.parametrize("mode", [0, 1], ids=["default", "expand"])
 def test_padded_taps_never_read_the_image(af, mode):
-    """The fused kernel rounds the tap count up to a bucket (4 -> 5) and gives
-    the extra tap weight 0 over zeroed halo cells: an Inf just outside the
-    filter's reach must not turn 0 * Inf into NaN anywhere the reference is
-    finite."""
+    """The fused kernel rounds the tap count up to a bucket (4 -> 5) and must
+    skip the taps past the filter rather than multiply them by 0: Inf in the
+    image must not turn 0 * Inf into NaN anywhere the reference is finite."""
     x = np.ones((128, 24), dtype=np.float32)
     x[66, :] = np.inf     # one column past what tile 0's outputs read
     x[:, 20] = np.inf
