@@ -56,6 +56,7 @@ struct PatchParams {
     int ngroups;         // ceil(RS / Q_TPS)
     int nseg;            // 16-pixel blocks per input row
     int items;           // 64 * nseg * PRmax 16-byte patch loads per stage
+    int toffs[9];        // patch offset of tap t (static group schedules, RS <= 9)
 };
 
 __device__ inline uint16_t pf32_to_f16_bits(float v) {
@@ -493,7 +494,15 @@ struct QTile {
     int id, n, kt, q0, q1, oh_a, ih0;
 };
 
-template <int MODE, typename TIn, typename TOut, int WM, int TPS, int PASS>
+template <int I, int N, typename F>
+__device__ __attribute__((always_inline)) inline void q_static_for(F &&f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        q_static_for<I + 1, N>(f);
+    }
+}
+
+template <int MODE, typename TIn, typename TOut, int WM, int TPS, int PASS, int NG>
 __global__ __launch_bounds__(Q_THREADS, 2) void conv_pipe(TOut *__restrict__ y,
                                                          const TIn *__restrict__ x,
                                                          const char *__restrict__ wpk,
@@ -526,7 +535,7 @@ __global__ __launch_bounds__(Q_THREADS, 2) void conv_pipe(TOut *__restrict__ y,
     for (int i = tid; i < 2 * patch_bytes / 16; i += Q_THREADS)
         reinterpret_cast<uint4 *>(lP)[i] = make_uint4(0, 0, 0, 0);
 
-    auto decode = [&](int id) {
+    auto decode = [&](int id) __attribute__((always_inline)) {
         QTile t;
         t.id = id;
         const int pt = id % p.tiles_per_img, r = id / p.tiles_per_img;
@@ -544,22 +553,29 @@ __global__ __launch_bounds__(Q_THREADS, 2) void conv_pipe(TOut *__restrict__ y,
     uint4 gx[Q::NSLOT];
     const int ch_l = g * SEG + (lane & (SEG - 1));
     const int px_l = (lane & 15) & ~(SEG - 1);
-    auto issue_gather = [&](const QTile &t, int cc) {
-        const TIn *xn = x + (long long)t.n * p.C * HW;
+    // Loads go through raw buffer resources: an offset at or beyond the
+    // resource's size reads zeros, so out-of-image segments and unused slots
+    // need neither a branch nor a zeroing move (either would make the
+    // compiler wait for every load in flight before reusing the registers).
+    // The host keeps one image (C*H*W elements) and the packed weights below
+    // 2^31 - 64 bytes.
+    constexpr int kOOB = 0x7fffffc0;
+    const int x_img_bytes = p.C * (int)HW * ESZ;
+    auto issue_gather = [&](const QTile &t, int cc, bool live) __attribute__((always_inline)) {
+        const auto xr = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<TIn *>(x + (long long)t.n * p.C * HW), (short)0, x_img_bytes, 0x00020000);
         const int c = cc * KC + ch_l;
 #pragma unroll
         for (int s = 0; s < Q::NSLOT; ++s) {
-            gx[s] = make_uint4(0, 0, 0, 0);
             const int rest = s * 8 + wave;
-            if (rest < nrest) {
-                const int row = rest / p.nseg, blk = rest - (rest / p.nseg) * p.nseg;
-                const int ih = t.ih0 + row, iw = blk * 16 + px_l;
-                if (ih >= 0 && ih < p.H && c < p.C && iw < p.W)
-                    gx[s] = *reinterpret_cast<const uint4 *>(xn + (long long)c * HW + (long long)ih * p.W + iw);
-            }
+            const int row = rest / p.nseg, blk = rest - (rest / p.nseg) * p.nseg;
+            const int ih = t.ih0 + row, iw = blk * 16 + px_l;
+            const bool ok = live && rest < nrest && ih >= 0 && ih < p.H && c < p.C && iw < p.W;
+            const int off = ok ? (c * (int)HW + ih * p.W + iw) * ESZ : kOOB;
+            gx[s] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(xr, off, 0, 0));
         }
     };
-    auto store_gather = [&](int buf) {
+    auto store_gather = [&](int buf) __attribute__((always_inline)) {
         char *dstb = lP + buf * patch_bytes;
 #pragma unroll
         for (int s = 0; s < Q::NSLOT; ++s) {
@@ -585,7 +601,7 @@ __global__ __launch_bounds__(Q_THREADS, 2) void conv_pipe(TOut *__restrict__ y,
     uint4 wa0 = make_uint4(0, 0, 0, 0), wa1 = wa0, wa2 = wa0, wb0 = wa0, wb1 = wa0, wb2 = wa0;
     int at = 0, acc_c = 0, ag = 0;        // cursor: tile step, chunk, group
     int akt = lb < ntiles ? (lb / p.tiles_per_img) / p.N : 0;
-    auto advance = [&]() {
+    auto advance = [&]() __attribute__((always_inline)) {
         if (++ag == p.ngroups) {
             ag = 0;
             if (++acc_c == p.nchunks) {
@@ -596,30 +612,36 @@ __global__ __launch_bounds__(Q_THREADS, 2) void conv_pipe(TOut *__restrict__ y,
             }
         }
     };
-    auto issue_A = [&](auto set) {  // loads the cursor's group, then advances it
+    // per-lane byte offset of weight slot s within a group (a = s*512 + tid;
+    // lanes past the group's end read zeros and store nothing)
+    const int w_total = p.ktiles * p.nchunks * p.RS * STEP;
+    int w_lane[3];
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        const int a = s * Q_THREADS + tid;
+        const int tl = a / (BM * 4), r = (a >> 2) % BM, q = a & 3;
+        w_lane[s] = a < TPS * BM * 4 ? tl * STEP + r * (KC * ESZ) + q * 16 : kOOB;
+    }
+    auto issue_A = [&](auto set) __attribute__((always_inline)) {  // loads the cursor's group, then advances it
         constexpr int S = decltype(set)::value;
         uint4 &r0 = S ? wb0 : wa0;
         uint4 &r1 = S ? wb1 : wa1;
         uint4 &r2 = S ? wb2 : wa2;
-        if (lb + at * G < ntiles) {
-            const char *src = wpk + ((long long)(akt * p.nchunks + acc_c) * p.RS + ag * TPS) * STEP;
-            constexpr int lim = TPS * BM * 4;
-            auto ld = [&](int s) {
-                const int a = s * Q_THREADS + tid;
-                const int tl = a / (BM * 4), r = (a >> 2) % BM, q = a & 3;
-                return a < lim ? *reinterpret_cast<const uint4 *>(src + tl * STEP + r * (KC * ESZ) + q * 16)
-                               : make_uint4(0, 0, 0, 0);
-            };
-            r0 = ld(0);
-            if constexpr (Q::ASLOT > 1) r1 = ld(1);
-            if constexpr (Q::ASLOT > 2) r2 = ld(2);
-        }
+        // the group's uniform offset goes into the resource (scalar work);
+        // a dead group gets an empty resource
+        const bool live = lb + at * G < ntiles;
+        const int src = ((akt * p.nchunks + acc_c) * p.RS + ag * TPS) * STEP;
+        const auto wr = __builtin_amdgcn_make_buffer_rsrc(const_cast<char *>(wpk) + src, (short)0,
+                                                          live ? w_total - src : 0, 0x00020000);
+        r0 = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(wr, w_lane[0], 0, 0));
+        if constexpr (Q::ASLOT > 1) r1 = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(wr, w_lane[1], 0, 0));
+        if constexpr (Q::ASLOT > 2) r2 = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(wr, w_lane[2], 0, 0));
         advance();
     };
-    auto store_A = [&](auto set, int buf) {
+    auto store_A = [&](auto set, int buf) __attribute__((always_inline)) {
         constexpr int S = decltype(set)::value;
         constexpr int lim = TPS * BM * 4;
-        auto st = [&](int s, const uint4 &v) {
+        auto st = [&](int s, const uint4 &v) __attribute__((always_inline)) {
             const int a = s * Q_THREADS + tid;
             const int tl = a / (BM * 4), r = (a >> 2) % BM, q = a & 3;
             if (a < lim) *reinterpret_cast<uint4 *>(lA + buf * Q::A_BUF + tl * BM * Q_PB + q_swz(r, q)) = v;
@@ -630,6 +652,7 @@ __global__ __launch_bounds__(Q_THREADS, 2) void conv_pipe(TOut *__restrict__ y,
     };
     using Set0 = std::integral_constant<int, 0>;
     using Set1 = std::integral_constant<int, 1>;
+    using FragT = typename std::conditional<MODE == PM_F32, float4, pu16x8>::type;
 
     // Accumulators: the MFMA's A operand is the patch (16 pixels x 32
     // channels) and B the weights, so acc[i][j][v] = out[pixel][filter] with
@@ -643,124 +666,84 @@ __global__ __launch_bounds__(Q_THREADS, 2) void conv_pipe(TOut *__restrict__ y,
 
     // weight fragment rows wm*64 + i*16 + lane%16: the swizzle key depends on lane%16 only
     const int abase = q_swz(wm * 64 + (lane & 15), g);
-    int k = 0;     // weight groups done (buffer parity)
-    int sbuf = 0;  // patch buffer of the current stage
-    QTile cur = decode(lb);
-    if (cur.id < ntiles) {
-        issue_A(Set0{});
-        issue_A(Set1{});
-        issue_gather(cur, 0);
-    }
-    __syncthreads();  // zeroed patch buffers visible
-    while (cur.id < ntiles) {
-        const QTile nxt = decode(cur.id + G);
-        const bool has_next = nxt.id < ntiles;
-        int bpos[4];  // patch position of this lane's pixel in fragment j (tap 0)
+    auto tile_bpos = [&](const QTile &t, int (&bp)[4]) __attribute__((always_inline)) {
 #pragma unroll
         for (int ni = 0; ni < 4; ++ni) {
-            int q = cur.q0 + wn * 64 + ni * 16 + (lane & 15);
-            if (q > cur.q1) q = cur.q1;  // dead pixels read a live one; never stored
+            int q = t.q0 + wn * 64 + ni * 16 + (lane & 15);
+            if (q > t.q1) q = t.q1;  // dead pixels read a live one; never stored
             const int oh = q / p.OW, ow = q - oh * p.OW;
-            bpos[ni] = ((oh - cur.oh_a) * p.s1) * p.PW + ow * p.s0;
+            bp[ni] = ((oh - t.oh_a) * p.s1) * p.PW + ow * p.s0;
         }
-        for (int cc = 0; cc < p.nchunks; ++cc) {
-            store_gather(sbuf);
-            const char *lp = lP + sbuf * patch_bytes;
-            for (int gq = 0; gq < p.ngroups; ++gq) {
-                // publish this group's weights (and, at gq == 0, the stage's
-                // patch), then refill the freed register set with group k + 2
-                if (k & 1) {
-                    store_A(Set1{}, 1);
-                    __syncthreads();
-                    issue_A(Set1{});
-                } else {
-                    store_A(Set0{}, 0);
-                    __syncthreads();
-                    issue_A(Set0{});
-                }
-                // next stage's patch: in flight for the whole stage
-                if (gq == 0) {
-                    const bool chunk = cc + 1 < p.nchunks;
-                    if (chunk || has_next) issue_gather(chunk ? cur : nxt, chunk ? cc + 1 : 0);
-                }
-                const char *aw = lA + (k & 1) * Q::A_BUF;
-                // TPS taps, straight-line: fragments of tap u + 1 are read
-                // from LDS while the MFMAs of tap u run
-                using FragT = typename std::conditional<MODE == PM_F32, float4, pu16x8>::type;
-                FragT af[2][4], bfr[2][4];
-                auto load_frags = [&](int u, FragT (&fa)[4], FragT (&fb)[4]) {
-                    const int tap = gq * TPS + u;
-                    const int ti = tap % p.fw, tj = tap / p.fw;
-                    const int toff = tj * p.d1 * p.PW + ti * p.d0;
-                    const char *au = aw + u * BM * Q_PB + abase;
+    };
+    // ---- epilogue of tile t: straight from the accumulators, 4 consecutive
+    // pixels of one filter row per lane and fragment (no LDS, no barrier).
+    // vec_out == 2 (4 | OH*OW, 16-byte aligned output, one output image under
+    // 2^31 - 64 bytes): one buffer store per fragment, dead lanes dropped by
+    // an offset past the resource's end; otherwise guarded stores.
+    const int y_lane = ((wm * 64 + (lane & 15)) * P + wn * 64 + 4 * (lane >> 4)) * (int)sizeof(TOut);
+    auto store_tile = [&](const QTile &t) __attribute__((always_inline)) {
+        if (vec_out == 2) {
+            const auto yr = __builtin_amdgcn_make_buffer_rsrc(y + (long long)t.n * p.K * (long long)P, (short)0,
+                                                              p.K * P * (int)sizeof(TOut), 0x00020000);
+            const int tofs = (t.kt * BM * P + t.q0) * (int)sizeof(TOut);
+            const int krem = p.K - t.kt * BM - wm * 64 - (lane & 15);   // filter rows left for fragment row 0
+            const int qrem = t.q1 - t.q0 - wn * 64 - 4 * (lane >> 4);   // pixels left for fragment column 0
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) fa[i] = *reinterpret_cast<const FragT *>(au + i * 16 * Q_PB);
+            for (int i = 0; i < 4; ++i)
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) fb[j] = *reinterpret_cast<const FragT *>(lp + q_swz(bpos[j] + toff, g));
-                };
-                load_frags(0, af[0], bfr[0]);
-#pragma unroll
-                for (int u = 0; u < TPS; ++u) {
-                    if (u + 1 < TPS) load_frags(u + 1, af[(u + 1) & 1], bfr[(u + 1) & 1]);
-                    const FragT(&fa)[4] = af[u & 1];
-                    const FragT(&fb)[4] = bfr[u & 1];
-#pragma unroll
-                    for (int i = 0; i < 4; ++i)
-#pragma unroll
-                        for (int j = 0; j < 4; ++j) {
-                            if constexpr (MODE == PM_F32) {
-                                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[j].x, fa[i].x, acc[i][j], 0, 0, 0);
-                                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[j].y, fa[i].y, acc[i][j], 0, 0, 0);
-                                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[j].z, fa[i].z, acc[i][j], 0, 0, 0);
-                                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[j].w, fa[i].w, acc[i][j], 0, 0, 0);
-                            } else if constexpr (MODE == PM_BF16) {
-                                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                                    __builtin_bit_cast(pbf16x8, fb[j]), __builtin_bit_cast(pbf16x8, fa[i]), acc[i][j], 0, 0, 0);
-                            } else {
-                                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-                                    __builtin_bit_cast(pf16x8, fb[j]), __builtin_bit_cast(pf16x8, fa[i]), acc[i][j], 0, 0, 0);
-                            }
+                for (int j = 0; j < 4; ++j) {
+                    const bool ok = i * 16 < krem && j * 16 <= qrem;
+                    const int off = ok ? y_lane + tofs + (i * 16 * P + j * 16) * (int)sizeof(TOut) : kOOB;
+                    const pf32x4 v = acc[i][j];
+                    if constexpr (sizeof(TOut) == 4) {
+                        __builtin_amdgcn_raw_buffer_store_b128(
+                            __builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned int, v), yr, off, 0, 0);
+                    } else {
+                        __attribute__((ext_vector_type(2))) unsigned int pk;
+                        if constexpr (MODE == PM_F16) {
+                            pk.x = (uint32_t)pf32_to_f16_bits(v[0]) | ((uint32_t)pf32_to_f16_bits(v[1]) << 16);
+                            pk.y = (uint32_t)pf32_to_f16_bits(v[2]) | ((uint32_t)pf32_to_f16_bits(v[3]) << 16);
+                        } else {
+                            pk.x = (uint32_t)f32_to_bf16_bits(v[0]) | ((uint32_t)f32_to_bf16_bits(v[1]) << 16);
+                            pk.y = (uint32_t)f32_to_bf16_bits(v[2]) | ((uint32_t)f32_to_bf16_bits(v[3]) << 16);
                         }
+                        __builtin_amdgcn_raw_buffer_store_b64(pk, yr, off, 0, 0);
+                    }
+                    acc[i][j] = pf32x4{0.f, 0.f, 0.f, 0.f};
                 }
-                ++k;
-            }
-            sbuf ^= 1;
+            return;
         }
-
-        // ---- epilogue: straight from the accumulators, 4 consecutive pixels
-        // of one filter row per lane and fragment (no LDS, no barrier; the
-        // next tile's first patch and weight groups are already in flight)
-        TOut *yn = y + (long long)cur.n * p.K * (long long)P;
+        TOut *yn = y + (long long)t.n * p.K * (long long)P;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int kf = cur.kt * BM + wm * 64 + i * 16 + (lane & 15);
+            const int kf = t.kt * BM + wm * 64 + i * 16 + (lane & 15);
             if (kf >= p.K) continue;
             TOut *yrow = yn + (long long)kf * P;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int q = cur.q0 + wn * 64 + j * 16 + 4 * (lane >> 4);
-                if (q > cur.q1) continue;
+                const int q = t.q0 + wn * 64 + j * 16 + 4 * (lane >> 4);
+                if (q > t.q1) continue;
                 const pf32x4 v = acc[i][j];
                 if constexpr (sizeof(TOut) == 4) {
-                    if (vec_out && q + 3 <= cur.q1) {
+                    if (vec_out && q + 3 <= t.q1) {
                         *reinterpret_cast<float4 *>(yrow + q) = make_float4(v[0], v[1], v[2], v[3]);
                     } else {
 #pragma unroll
                         for (int e = 0; e < 4; ++e)
-                            if (q + e <= cur.q1) yrow[q + e] = v[e];
+                            if (q + e <= t.q1) yrow[q + e] = v[e];
                     }
                 } else {
                     uint16_t hb[4];
 #pragma unroll
                     for (int e = 0; e < 4; ++e) hb[e] = MODE == PM_F16 ? pf32_to_f16_bits(v[e]) : f32_to_bf16_bits(v[e]);
                     uint16_t *d16 = reinterpret_cast<uint16_t *>(yrow) + q;
-                    if (vec_out && q + 3 <= cur.q1) {
+                    if (vec_out && q + 3 <= t.q1) {
                         *reinterpret_cast<uint2 *>(d16) =
                             make_uint2((uint32_t)hb[0] | ((uint32_t)hb[1] << 16), (uint32_t)hb[2] | ((uint32_t)hb[3] << 16));
                     } else {
 #pragma unroll
                         for (int e = 0; e < 4; ++e)
-                            if (q + e <= cur.q1) d16[e] = hb[e];
+                            if (q + e <= t.q1) d16[e] = hb[e];
                     }
                 }
             }
@@ -769,7 +752,176 @@ __global__ __launch_bounds__(Q_THREADS, 2) void conv_pipe(TOut *__restrict__ y,
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j) acc[i][j] = pf32x4{0.f, 0.f, 0.f, 0.f};
-        cur = nxt;
+    };
+    // MFMAs of one tap from fragments fa (weights) and fb (patch)
+    auto mma_tap = [&](const FragT (&fa)[4], const FragT (&fb)[4]) __attribute__((always_inline)) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if constexpr (MODE == PM_F32) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[j].x, fa[i].x, acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[j].y, fa[i].y, acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[j].z, fa[i].z, acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[j].w, fa[i].w, acc[i][j], 0, 0, 0);
+                } else if constexpr (MODE == PM_BF16) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(pbf16x8, fb[j]), __builtin_bit_cast(pbf16x8, fa[i]), acc[i][j], 0, 0, 0);
+                } else {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                        __builtin_bit_cast(pf16x8, fb[j]), __builtin_bit_cast(pf16x8, fa[i]), acc[i][j], 0, 0, 0);
+                }
+            }
+    };
+    // fragments of one tap: weights from buffer aw (tap row block u), patch
+    // from lp at pixel positions bp + to
+    auto load_tap = [&](const char *aw, int u, const char *lp, const int (&bp)[4], int to, FragT (&fa)[4],
+                        FragT (&fb)[4]) __attribute__((always_inline)) {
+        const char *au = aw + u * BM * Q_PB + abase;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) fa[i] = *reinterpret_cast<const FragT *>(au + i * 16 * Q_PB);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) fb[j] = *reinterpret_cast<const FragT *>(lp + q_swz(bp[j] + to, g));
+    };
+
+
+    int sbuf = 0;         // patch buffer of the current stage
+    int cc = 0, gq = 0;   // stage (channel chunk) and weight group within the current tile
+    QTile cur = decode(lb);
+    if (cur.id < ntiles) {
+        issue_A(Set0{});
+        issue_A(Set1{});
+        issue_gather(cur, 0, true);
+    }
+    __syncthreads();  // zeroed patch buffers visible
+    if (cur.id >= ntiles) return;  // uniform: lb is the same for the whole workgroup
+    QTile nxt = decode(cur.id + G);
+
+    // Patch fragment addresses. Runtime schedule: q_swz(bpos[j] + tap
+    // offset) per tap. Static 3 x 3 schedule (fw = 3, d0 = 1, d1*PW a
+    // multiple of 8, checked on the host): group gq holds filter row gq and
+    // taps u = 0..2 of it; a row step of d1*PW pixels keeps the swizzle key
+    // ((p >> 1) & 3), so the 3 column phases are swizzled once per tile and
+    // a tap's address is pb[j][u] + gq * d1*PW*64.
+    int bpos[4];      // patch position of this lane's pixel in fragment j (tap 0)
+    int pb[4][3];     // static 3 x 3 schedule: swizzled address of fragment j, column phase u
+    auto set_tile = [&]() __attribute__((always_inline)) {
+        tile_bpos(cur, bpos);
+        if constexpr (NG == 3) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int u = 0; u < 3; ++u) pb[j][u] = q_swz(bpos[j] + u, g);
+        }
+    };
+    set_tile();
+    const int rowb = p.d1 * p.PW * Q_PB;  // bytes per filter-row step in the patch
+
+    // Start of a weight group with register set / LDS weight buffer S: at the
+    // first group of a stage publish the stage's patch, then publish the
+    // group's weights, barrier, refill the set with group k + 2 and (first
+    // group) start the next stage's patch. Loads are issued unconditionally
+    // (dead ones read zeros), so the number of loads between a load and its
+    // use is the same on every path and the compiler waits only for it.
+    auto begin_group = [&](auto set, auto first) __attribute__((always_inline)) {
+        constexpr int S = decltype(set)::value;
+        if (static_cast<bool>(first)) store_gather(sbuf);
+        store_A(set, S);
+        __syncthreads();
+        issue_A(set);
+        if (static_cast<bool>(first)) {
+            const bool chunk = cc + 1 < p.nchunks;
+            issue_gather(chunk ? cur : nxt, chunk ? cc + 1 : 0, chunk || nxt.id < ntiles);
+        }
+    };
+
+    // TPS taps from weight buffer aw and patch lp, straight-line: fragments
+    // of tap u + 1 are read from LDS while the MFMAs of tap u run.
+    // paddr(u, j) is the patch byte offset of fragment j of the group's tap u.
+    auto mma_group = [&](const char *aw, const char *lp, auto paddr) __attribute__((always_inline)) {
+        FragT af[2][4], bfr[2][4];
+        auto load_frags = [&](int u, FragT (&fa)[4], FragT (&fb)[4]) __attribute__((always_inline)) {
+            const char *au = aw + u * BM * Q_PB + abase;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) fa[i] = *reinterpret_cast<const FragT *>(au + i * 16 * Q_PB);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) fb[j] = *reinterpret_cast<const FragT *>(lp + paddr(u, j));
+        };
+        load_frags(0, af[0], bfr[0]);
+#pragma unroll
+        for (int u = 0; u < TPS; ++u) {
+            if (u + 1 < TPS) load_frags(u + 1, af[(u + 1) & 1], bfr[(u + 1) & 1]);
+            mma_tap(af[u & 1], bfr[u & 1]);
+        }
+    };
+
+    // End of a stage: flip the patch buffer; after the last stage of a tile
+    // store it and move to the next. Returns true when no tile is left.
+    auto end_stage = [&]() __attribute__((always_inline)) -> bool {
+        sbuf ^= 1;
+        if (++cc == p.nchunks) {
+            cc = 0;
+            store_tile(cur);
+            cur = nxt;
+            if (cur.id >= ntiles) return true;
+            nxt = decode(cur.id + G);
+            set_tile();
+        }
+        return false;
+    };
+
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    if constexpr (NG > 0) {
+        // static schedule: NG groups per stage (p.ngroups == NG), unrolled, so
+        // every register set, group index and tap offset is a constant; two
+        // stages per iteration when NG is odd (the set parity alternates)
+        auto stage = [&](auto par) __attribute__((always_inline)) -> bool {
+            constexpr int P0 = decltype(par)::value;
+            q_static_for<0, NG>([&](auto gi) __attribute__((always_inline)) {
+                constexpr int GI = decltype(gi)::value;
+                constexpr int S = (P0 + GI) & 1;
+                begin_group(std::integral_constant<int, S>{}, std::integral_constant<bool, GI == 0>{});
+                if constexpr (NG == 3) {
+                    mma_group(lA + S * Q::A_BUF, lP + sbuf * patch_bytes + GI * rowb,
+                              [&](int u, int j) __attribute__((always_inline)) { return pb[j][u]; });
+                } else {
+                    mma_group(lA + S * Q::A_BUF, lP + sbuf * patch_bytes,
+                              [&](int u, int j) __attribute__((always_inline)) {
+                                  return q_swz(bpos[j] + p.toffs[GI * TPS + u], g);
+                              });
+                }
+            });
+            return end_stage();
+        };
+        if constexpr (NG % 2 == 0) {
+            while (!stage(I0{})) {
+            }
+        } else {
+            while (!stage(I0{}) && !stage(I1{})) {
+            }
+        }
+    } else {
+        // runtime schedule: the loop alternates register sets S = 0, 1, so
+        // each set stays a fixed group of registers (a copy between sets
+        // would wait for the loads at once)
+        auto group = [&](auto set) __attribute__((always_inline)) -> bool {
+            constexpr int S = decltype(set)::value;
+            begin_group(set, gq == 0);
+            const int tap0 = gq * TPS;
+            mma_group(lA + S * Q::A_BUF, lP + sbuf * patch_bytes, [&](int u, int j) __attribute__((always_inline)) {
+                const int tap = tap0 + u;
+                const int ti = tap % p.fw, tj = tap / p.fw;
+                return q_swz(bpos[j] + tj * p.d1 * p.PW + ti * p.d0, g);
+            });
+            if (++gq == p.ngroups) {
+                gq = 0;
+                return end_stage();
+            }
+            return false;
+        };
+        while (!group(Set0{}) && !group(Set1{})) {
+        }
     }
 }
 
@@ -835,6 +987,12 @@ static bool plan_pipe(PatchParams &q, const PatchParams &p, const void *x, int &
         const int nslot = wm == 2 ? QCfg<MODE, 2>::NSLOT : QCfg<MODE, 1>::NSLOT;
         if ((long long)q.nseg * q.PRmax > 8LL * nslot || q.ntiles >= (1LL << 31) - (1LL << 20)) return false;
         q.items = 64 * q.nseg * q.PRmax;
+        // buffer-resource offsets are 32-bit: one image and the packed
+        // weights must stay below 2^31 - 64 bytes
+        const long long img_bytes = (long long)q.C * q.H * q.W * (long long)sizeof(TIn);
+        const long long w_bytes = (long long)q.ktiles * q.nchunks * q.RS * P_BM * PCfg<MODE>::KC * PCfg<MODE>::ESZ;
+        if (img_bytes >= (1LL << 31) - 64 || w_bytes >= (1LL << 31) - 64) return false;
+        for (int t = 0; t < 9; ++t) q.toffs[t] = t < q.RS ? (t / q.fw) * q.d1 * q.PW + (t % q.fw) * q.d0 : 0;
         const size_t lds = (size_t)2 * Q_TPS * BM * Q_PB + (size_t)2 * q.PRmax * q.PW * Q_PB;
         return lds <= kPatchLdsLimit;
     }
@@ -856,18 +1014,30 @@ template <int MODE, typename TIn, typename TOut>
 static int launch_pipe(void *y, const void *x, const void *ws, const PatchParams &q, int wm, int dgrad,
                        hipStream_t s) {
     const int BM = 64 * wm;
-    const size_t lds = (size_t)2 * Q_TPS * BM * Q_PB + (size_t)2 * q.PRmax * q.PW * Q_PB;
-    using Fn = void (*)(TOut *, const TIn *, const char *, PatchParams, int);
+    const size_t patch = (size_t)2 * q.PRmax * q.PW * Q_PB;
+    // schedule: TPS = 3 taps per weight group when taps % 3 == 0, else 1;
+    // a static (unrolled) group schedule for 3 x 3 taps (3 groups of 3, when
+    // a filter-row step keeps the patch swizzle) and for 1 tap (1 group of
+    // 1), a runtime one otherwise
     const bool t3 = q.RS % Q_TPS == 0;
+    const bool s33 = q.RS == 9 && q.fw == 3 && q.d0 == 1 && (q.d1 * q.PW) % 8 == 0;
+    const int sched = t3 ? (s33 ? 0 : 1) : (q.RS == 1 ? 2 : 3);
+    const size_t lds = (size_t)2 * Q_TPS * BM * Q_PB + patch;
+    const int P = q.OH * q.OW;
+    using Fn = void (*)(TOut *, const TIn *, const char *, PatchParams, int);
     Fn fn;
+#define AFHIP_PIPE_FN(WMv, DG)                                                                     \
+    (sched == 0   ? conv_pipe<MODE, TIn, TOut, WMv, 3, DG, 3>                                     \
+     : sched == 1 ? conv_pipe<MODE, TIn, TOut, WMv, 3, DG, 0>                                     \
+     : sched == 2 ? conv_pipe<MODE, TIn, TOut, WMv, 1, DG, 1>                                     \
+                  : conv_pipe<MODE, TIn, TOut, WMv, 1, DG, 0>)
     if (wm == 1)
-        fn = t3 ? (dgrad ? conv_pipe<MODE, TIn, TOut, 1, 3, 1> : conv_pipe<MODE, TIn, TOut, 1, 3, 0>)
-                : (dgrad ? conv_pipe<MODE, TIn, TOut, 1, 1, 1> : conv_pipe<MODE, TIn, TOut, 1, 1, 0>);
+        fn = dgrad ? AFHIP_PIPE_FN(1, 1) : AFHIP_PIPE_FN(1, 0);
     else
-        fn = t3 ? (dgrad ? conv_pipe<MODE, TIn, TOut, 2, 3, 1> : conv_pipe<MODE, TIn, TOut, 2, 3, 0>)
-                : (dgrad ? conv_pipe<MODE, TIn, TOut, 2, 1, 1> : conv_pipe<MODE, TIn, TOut, 2, 1, 0>);
-    const int slot = ((wm - 1) * 2 + dgrad) * 2 + (t3 ? 1 : 0);
-    static bool attr_set[8] = {false, false, false, false, false, false, false, false};
+        fn = dgrad ? AFHIP_PIPE_FN(2, 1) : AFHIP_PIPE_FN(2, 0);
+#undef AFHIP_PIPE_FN
+    const int slot = ((wm - 1) * 2 + dgrad) * 4 + sched;
+    static bool attr_set[16] = {false};
     if (!attr_set[slot]) {
         if (hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)kPatchLdsLimit) != hipSuccess)
@@ -881,8 +1051,10 @@ static int launch_pipe(void *y, const void *x, const void *ws, const PatchParams
         per_cu = 1;
     long long grid = (long long)device_cu_count() * per_cu;
     if (grid > q.ntiles) grid = q.ntiles;
-    const int P = q.OH * q.OW;
-    const int vec_out = (P % 4 == 0) && ((reinterpret_cast<uintptr_t>(y) & 15) == 0);
+    // 1: whole 4-pixel groups and an aligned output; 2: also one output image
+    // addressable with 32-bit buffer offsets
+    int vec_out = (P % 4 == 0) && ((reinterpret_cast<uintptr_t>(y) & 15) == 0);
+    if (vec_out && (long long)q.K * P * (long long)sizeof(TOut) < (1LL << 31) - 64) vec_out = 2;
     fn<<<dim3((unsigned)grid), Q_THREADS, lds, s>>>((TOut *)y, (const TIn *)x, (const char *)ws, q, vec_out);
     return launch_status();
 }
