@@ -31,6 +31,7 @@ typedef uint32_t wu32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int WG_BK = 64, WG_BC = 64, WG_THREADS = 256;
 constexpr size_t WG_LDS_LIMIT = 150 * 1024;
+constexpr size_t WG_PC_LDS = 160 * 1024;  // two block buffers of the producer/consumer kernel
 
 struct WGParams {
     int W, H, C, N, OW, OH, K;
@@ -41,6 +42,10 @@ struct WGParams {
     int units_per_img, total_units, nsplit, upsplit;
     int ktiles, ctiles, RS;
     int is_f16;
+    int pc;              // 1: producer/consumer kernel with two block buffers
+    int bufb;            // bytes per block buffer (16-byte multiple)
+    int dy_lpr_log2;     // producer copy: log2 of lanes per dy row (>= segrow)
+    int x_rpr;           // producer copy: x rows per round (256 / (PWpad / 8))
 };
 
 __device__ inline uint4 pack8(const uint16_t *h) {
@@ -97,12 +102,12 @@ __device__ inline uint4 x_chunk(const uint16_t *__restrict__ x, const WGParams &
     }
 }
 
-template <bool ALIGNED>
+template <bool ALIGNED, int NT = WG_THREADS, int NBA = 8>
 __device__ inline void load_direct(const uint16_t *__restrict__ x, const uint16_t *__restrict__ dy,
                                    uint16_t *lds_dy, uint16_t *lds_x, const WGParams &p, int n, int oh0,
-                                   int kt, int ct) {
-    constexpr int NB = ALIGNED ? 8 : 4;
-    const int tid = threadIdx.x;
+                                   int kt, int ct, int tid = (int)threadIdx.x) {
+    constexpr int NB = ALIGNED ? NBA : 4;
+    constexpr int WG_THREADS = NT;  // threads taking part in the copy
     const long long HW = (long long)p.H * p.W, OHW = (long long)p.OH * p.OW;
     const int segs = p.steps * 4;
     const int dchunks = WG_BK * segs;
@@ -268,6 +273,209 @@ __global__ __launch_bounds__(WG_THREADS, 2) void wgrad_rows(float *__restrict__ 
         }
 }
 
+// Producer copy of one block for ALIGNED geometries (W and OW multiples of
+// 8): 256 lanes laid out as rows x 16-byte chunks. A lane's row advances by a
+// constant stride each round (no division), addresses are 32-bit offsets into
+// one image's buffer resource, and out-of-image chunks read zeros (offset past
+// the resource's end), so each chunk costs a few VALU operations. NB rounds of
+// loads are in flight before their LDS stores. The dy chunks of dummy
+// segments (nseg..steps*4) are never written: they are zeroed once.
+template <int NB>
+__device__ inline void pc_copy(const uint16_t *__restrict__ x, const uint16_t *__restrict__ dy, uint16_t *ldy,
+                               uint16_t *lx, const WGParams &p, int n, int oh0, int kt, int ct, int t) {
+    constexpr int kOOB = 0x7fffffc0;
+    // ---- dy: WG_BK * R_oh rows (filter m, output row r) of segrow chunks
+    {
+        const auto rs = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<uint16_t *>(dy + (long long)n * p.K * p.OH * p.OW), (short)0, p.K * p.OH * p.OW * 2, 0x00020000);
+        const int rpr = WG_THREADS >> p.dy_lpr_log2;  // rows per round
+        const int j = t & ((1 << p.dy_lpr_log2) - 1);
+        const int nrow = WG_BK * p.R_oh;
+        const int row0 = t >> p.dy_lpr_log2;
+        int m = row0 / p.R_oh, r = row0 - (row0 / p.R_oh) * p.R_oh;
+        const int dm = rpr / p.R_oh, dr = rpr - (rpr / p.R_oh) * p.R_oh;
+        const bool jok = j < p.segrow;
+        for (int base = 0; base < nrow; base += NB * rpr) {
+            uint4 v[NB];
+            int dst[NB];
+#pragma unroll
+            for (int q = 0; q < NB; ++q) {
+                const int k = kt * WG_BK + m, oh = oh0 + r;
+                const bool in = base + q * rpr + row0 < nrow && jok;
+                const bool ok = in && k < p.K && oh < p.OH;
+                v[q] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(
+                                                     rs, ok ? ((k * p.OH + oh) * p.OW + j * 8) * 2 : kOOB, 0, 0));
+                dst[q] = in ? m * p.dyp + (r * p.segrow + j) * 8 : -1;
+                m += dm;
+                r += dr;
+                if (r >= p.R_oh) {
+                    r -= p.R_oh;
+                    ++m;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < NB; ++q)
+                if (dst[q] >= 0) *reinterpret_cast<uint4 *>(ldy + dst[q]) = v[q];
+        }
+    }
+    // ---- x: WG_BC * PR rows (channel c, patch row r) of qrow chunks, qrow
+    // lanes per row, x_rpr rows per round (lanes past x_rpr * qrow idle)
+    {
+        const auto rs = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<uint16_t *>(x + (long long)n * p.C * p.H * p.W), (short)0, p.C * p.H * p.W * 2, 0x00020000);
+        const int qrow = p.PWpad / 8, rpr = p.x_rpr;
+        const int qq = t % qrow;
+        const int nrow = WG_BC * p.PR;
+        const int row0 = t / qrow;
+        const bool lane_ok = row0 < rpr;
+        int c = row0 / p.PR, r = row0 - (row0 / p.PR) * p.PR;
+        const int dc = rpr / p.PR, dr = rpr - (rpr / p.PR) * p.PR;
+        const int iw0 = qq * 8 - p.PADL;
+        const bool wok = iw0 >= 0 && iw0 < p.W;  // a chunk never straddles a row end (W % 8 == 0)
+        const int ih00 = oh0 * p.s1 - p.p1;
+        for (int base = 0; base < nrow; base += NB * rpr) {
+            uint4 v[NB];
+            int dst[NB];
+#pragma unroll
+            for (int q = 0; q < NB; ++q) {
+                const int row = base + q * rpr + row0;
+                const int ch = ct * WG_BC + c, ih = ih00 + r;
+                const bool in = lane_ok && row < nrow;
+                const bool ok = in && wok && ch < p.C && ih >= 0 && ih < p.H;
+                v[q] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(
+                                                     rs, ok ? ((ch * p.H + ih) * p.W + iw0) * 2 : kOOB, 0, 0));
+                dst[q] = in ? row * p.PWpad + qq * 8 : -1;
+                c += dc;
+                r += dr;
+                if (r >= p.PR) {
+                    r -= p.PR;
+                    ++c;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < NB; ++q)
+                if (dst[q] >= 0) *reinterpret_cast<uint4 *>(lx + dst[q]) = v[q];
+        }
+    }
+}
+
+// Producer/consumer form of wgrad_rows: 8 waves, the first 4 compute exactly
+// as in wgrad_rows, the other 4 copy the next block (dy rows and the x rows
+// they touch) into the second of two LDS block buffers meanwhile, with 16
+// chunks per lane in flight. One barrier per block hands the buffers over,
+// so HBM reads overlap the MFMAs instead of alternating with them.
+template <int FW, int FH, int S0, int F16, bool ALIGNED>
+__global__ __launch_bounds__(2 * WG_THREADS, 2) void wgrad_pc(float *__restrict__ out,
+                                                              const uint16_t *__restrict__ x,
+                                                              const uint16_t *__restrict__ dy,
+                                                              WGParams p) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const bool producer = wave >= 4;
+    long long bid = blockIdx.x;
+    const int split = (int)(bid % p.nsplit);
+    bid /= p.nsplit;
+    const int ct = (int)(bid % p.ctiles);
+    const int kt = (int)(bid / p.ctiles);
+    const int u0 = split * p.upsplit;
+    const int u1 = min(u0 + p.upsplit, p.total_units);
+    uint16_t *const lds_dy0 = reinterpret_cast<uint16_t *>(smem);
+    uint16_t *const lds_dy1 = reinterpret_cast<uint16_t *>(smem + p.bufb);
+    const int xoff = WG_BK * p.dyp;  // x block after the dy block, in elements
+    // Producers and consumers run separate loops with the same barriers (one
+    // before the first block, one after each block), so the accumulators are
+    // live only on the consumer side.
+    if (producer) {
+        const int t = tid - WG_THREADS;
+        if constexpr (ALIGNED) {
+            // dummy dy segments (nseg .. steps*4) of both buffers: zero once
+            const int dseg = p.steps * 4 - p.nseg;
+            for (int e = t; e < 2 * WG_BK * dseg; e += WG_THREADS) {
+                const int bsel = e / (WG_BK * dseg), rem = e - bsel * (WG_BK * dseg);
+                const int m = rem / dseg, sg = p.nseg + (rem - (rem / dseg) * dseg);
+                *reinterpret_cast<uint4 *>((bsel ? lds_dy1 : lds_dy0) + m * p.dyp + sg * 8) = make_uint4(0, 0, 0, 0);
+            }
+        }
+        for (int i = -1; u0 + i < u1; ++i) {
+            const int u = u0 + i + 1;
+            if (u < u1) {
+                uint16_t *ldy = (i & 1) ? lds_dy0 : lds_dy1;  // block i + 1 goes to buffer (i + 1) & 1
+                const int n = u / p.units_per_img;
+                if constexpr (ALIGNED)
+                    pc_copy<8>(x, dy, ldy, ldy + xoff, p, n, (u - n * p.units_per_img) * p.R_oh, kt, ct, t);
+                else
+                    load_direct<ALIGNED, WG_THREADS, 16>(x, dy, ldy, ldy + xoff, p, n,
+                                                         (u - n * p.units_per_img) * p.R_oh, kt, ct, t);
+            }
+            __syncthreads();  // block i consumed (i >= 0), block i + 1 published
+        }
+        return;
+    }
+
+    constexpr int RS = FW * FH;
+    wf32x4 acc[4][RS];
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+        for (int t = 0; t < RS; ++t) acc[mi][t] = wf32x4{0.f, 0.f, 0.f, 0.f};
+
+    const int g = lane >> 4, r16 = lane & 15;
+    const int cl = wave * 16 + r16;  // lane's channel within the tile
+    __syncthreads();
+    for (int i = 0; u0 + i < u1; ++i) {
+        const uint16_t *ldy = (i & 1) ? lds_dy1 : lds_dy0;
+        const uint16_t *lx = ldy + xoff;
+        for (int s = 0; s < p.steps; ++s) {
+            int seg = s * 4 + g;
+            wu32x4 af[4];
+#pragma unroll
+            for (int mi = 0; mi < 4; ++mi)
+                af[mi] = *reinterpret_cast<const wu32x4 *>(ldy + (mi * 16 + r16) * p.dyp + seg * 8);
+            if (seg >= p.nseg) seg = p.nseg - 1;  // dummy segment: A is zero there
+            const int ro = seg / p.segrow;
+            const int ow0 = (seg - ro * p.segrow) * 8;
+            const int e0 = ow0 * S0 + (p.PADL - p.p0);
+#pragma unroll
+            for (int j = 0; j < FH; ++j) {
+                const uint16_t *rowp = lx + (cl * p.PR + ro * p.s1 + j * p.d1) * p.PWpad;
+                wu32x4 bf[FW];
+                b_frags<FW, S0>(rowp, e0, bf);
+#pragma unroll
+                for (int ii = 0; ii < FW; ++ii)
+#pragma unroll
+                    for (int mi = 0; mi < 4; ++mi) {
+                        if constexpr (F16)
+                            acc[mi][j * FW + ii] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                                __builtin_bit_cast(wf16x8, af[mi]), __builtin_bit_cast(wf16x8, bf[ii]),
+                                acc[mi][j * FW + ii], 0, 0, 0);
+                        else
+                            acc[mi][j * FW + ii] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                __builtin_bit_cast(wbf16x8, af[mi]), __builtin_bit_cast(wbf16x8, bf[ii]),
+                                acc[mi][j * FW + ii], 0, 0, 0);
+                    }
+            }
+        }
+        __syncthreads();  // block i consumed, block i + 1 published
+    }
+
+    // acc[mi][t][v] = D[filter][channel] for tap t = j*FW + i:
+    // filter = kt*64 + mi*16 + 4*(lane/16) + v, channel = ct*64 + wave*16 + lane%16.
+    const int c = ct * WG_BC + cl;
+    if (c >= p.C) return;
+    const long long total = (long long)p.K * p.C * p.RS;
+    float *dst = out + (long long)split * total;
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int k = kt * WG_BK + mi * 16 + g * 4 + v;
+            if (k >= p.K) continue;
+            float *drow = dst + ((long long)k * p.C + c) * p.RS;
+#pragma unroll
+            for (int t = 0; t < RS; ++t) drow[RS - 1 - t] = acc[mi][t][v];
+        }
+}
+
 // Sums nsplit partial dW buffers and converts to the output type. The order
 // of the additions is fixed (groups of 8 splits, then the groups in order),
 // so results do not depend on scheduling. Eight partials are in flight per
@@ -325,17 +533,31 @@ static bool wg_plan(WGParams &p, int fw, int s0, int d0) {
     p.PWpad = ((maxe + 7) & ~7) + 8;
     if (p.PADL - p.p0 < 0) return false;
     // rows per block: up to about 512 padded pixels, within the LDS budget
-    // (one workgroup per CU: taller blocks re-read fewer halo rows of x)
-    int r = 512 / p.OWp;
-    if (r < 1) r = 1;
-    if (r > p.OH) r = p.OH;
-    for (; r >= 1; --r) {
-        p.R_oh = r;
-        p.nseg = r * p.segrow;
-        p.steps = (p.nseg + 3) / 4;
-        p.dyp = p.steps * 32 + 8;
-        p.PR = (r - 1) * p.s1 + (p.fh - 1) * p.d1 + 1;
-        if (wg_lds(p) <= WG_LDS_LIMIT) break;
+    // (one workgroup per CU: taller blocks re-read fewer halo rows of x).
+    // First choice: the producer/consumer kernel, two block buffers in 160 KB;
+    // otherwise one buffer in WG_LDS_LIMIT.
+    int r0 = 512 / p.OWp;
+    if (r0 < 1) r0 = 1;
+    if (r0 > p.OH) r0 = p.OH;
+    // producer copy layout (pc_copy): lanes per dy row a power of two >= segrow,
+    // x rows per round; one image of x and of dy within 32-bit byte offsets
+    p.dy_lpr_log2 = 0;
+    while ((1 << p.dy_lpr_log2) < p.segrow) ++p.dy_lpr_log2;
+    p.x_rpr = WG_THREADS / (p.PWpad / 8);
+    const bool pc_ok = p.dy_lpr_log2 <= 8 && p.x_rpr >= 1 && (long long)p.C * p.H * p.W * 2 < (1LL << 31) - 64 &&
+                       (long long)p.K * p.OH * p.OW * 2 < (1LL << 31) - 64;
+    int r = 0;
+    for (int pc = pc_ok ? 1 : 0; pc >= 0 && r < 1; --pc) {
+        for (r = r0; r >= 1; --r) {
+            p.R_oh = r;
+            p.nseg = r * p.segrow;
+            p.steps = (p.nseg + 3) / 4;
+            p.dyp = p.steps * 32 + 8;
+            p.PR = (r - 1) * p.s1 + (p.fh - 1) * p.d1 + 1;
+            p.pc = pc;
+            p.bufb = (int)((wg_lds(p) + 15) & ~(size_t)15);
+            if (pc ? 2 * (size_t)p.bufb <= WG_PC_LDS : wg_lds(p) <= WG_LDS_LIMIT) break;
+        }
     }
     if (r < 1) return false;
     p.units_per_img = (p.OH + p.R_oh - 1) / p.R_oh;
@@ -370,6 +592,20 @@ static void wg_fill(WGParams &p, const long long *xd, const long long *wd, const
 template <int FW, int FH, int S0, int F16>
 static int wg_launch(float *part, const void *x, const void *dy, const WGParams &p, hipStream_t s) {
     const bool aligned = (p.W & 7) == 0 && (p.OW & 7) == 0;
+    if (p.pc) {
+        auto fp = aligned ? wgrad_pc<FW, FH, S0, F16, true> : wgrad_pc<FW, FH, S0, F16, false>;
+        static bool attr_pc[2] = {false, false};
+        if (!attr_pc[aligned]) {
+            if (hipFuncSetAttribute((const void *)fp, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)WG_PC_LDS) != hipSuccess)
+                return K_ERR_RUNTIME;
+            attr_pc[aligned] = true;
+        }
+        const long long grid = (long long)p.ktiles * p.ctiles * p.nsplit;
+        fp<<<dim3((unsigned)grid), 2 * WG_THREADS, 2 * (size_t)p.bufb, s>>>(part, (const uint16_t *)x,
+                                                                          (const uint16_t *)dy, p);
+        return launch_status();
+    }
     auto fn = aligned ? wgrad_rows<FW, FH, S0, F16, true> : wgrad_rows<FW, FH, S0, F16, false>;
     static bool attr[2] = {false, false};
     if (!attr[aligned]) {
