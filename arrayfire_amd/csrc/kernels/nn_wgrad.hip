@@ -257,20 +257,16 @@ __global__ __launch_bounds__(WG_THREADS, 2) void wgrad_rows(float *__restrict__ 
 
     // acc[mi][t][v] = D[filter][channel] for tap t = j*FW + i:
     // filter = kt*64 + mi*16 + 4*(lane/16) + v, channel = ct*64 + wave*16 + lane%16.
-    const int c = ct * WG_BC + cl;
-    if (c >= p.C) return;
-    const long long total = (long long)p.K * p.C * p.RS;
-    float *dst = out + (long long)split * total;
+    // Stored in the lane-contiguous partial layout (one 256-byte store per
+    // (mi, t, v)); wgrad_sum maps it to dW.
+    float *dst = out + ((long long)split * p.ktiles * p.ctiles + kt * p.ctiles + ct) * (4096LL * RS) +
+                 (long long)wave * (1024 * RS) + lane;
 #pragma unroll
     for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int k = kt * WG_BK + mi * 16 + g * 4 + v;
-            if (k >= p.K) continue;
-            float *drow = dst + ((long long)k * p.C + c) * p.RS;
+        for (int t = 0; t < RS; ++t)
 #pragma unroll
-            for (int t = 0; t < RS; ++t) drow[RS - 1 - t] = acc[mi][t][v];
-        }
+            for (int v = 0; v < 4; ++v) dst[((mi * RS + t) * 4 + v) * 64] = acc[mi][t][v];
 }
 
 // Producer copy of one block for ALIGNED geometries (W and OW multiples of
@@ -460,26 +456,24 @@ __global__ __launch_bounds__(2 * WG_THREADS, 2) void wgrad_pc(float *__restrict_
 
     // acc[mi][t][v] = D[filter][channel] for tap t = j*FW + i:
     // filter = kt*64 + mi*16 + 4*(lane/16) + v, channel = ct*64 + wave*16 + lane%16.
-    const int c = ct * WG_BC + cl;
-    if (c >= p.C) return;
-    const long long total = (long long)p.K * p.C * p.RS;
-    float *dst = out + (long long)split * total;
+    // Stored in the lane-contiguous partial layout (one 256-byte store per
+    // (mi, t, v)); wgrad_sum maps it to dW.
+    float *dst = out + ((long long)split * p.ktiles * p.ctiles + kt * p.ctiles + ct) * (4096LL * RS) +
+                 (long long)wave * (1024 * RS) + lane;
 #pragma unroll
     for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int k = kt * WG_BK + mi * 16 + g * 4 + v;
-            if (k >= p.K) continue;
-            float *drow = dst + ((long long)k * p.C + c) * p.RS;
+        for (int t = 0; t < RS; ++t)
 #pragma unroll
-            for (int t = 0; t < RS; ++t) drow[RS - 1 - t] = acc[mi][t][v];
-        }
+            for (int v = 0; v < 4; ++v) dst[((mi * RS + t) * 4 + v) * 64] = acc[mi][t][v];
 }
 
 // Sums nsplit partial dW buffers and converts to the output type. The order
 // of the additions is fixed (groups of 8 splits, then the groups in order),
 // so results do not depend on scheduling. Eight partials are in flight per
-// lane.
+// lane. A split's partials are tiles (kt, ct) of [wave][mi][tap][v][lane]
+// floats (4096 * RS per tile): thread e reads element e of every split
+// (coalesced) and writes the dW element it maps to.
 __device__ inline float sum_splits(const float *__restrict__ part, long long total, long long e,
                                    int nsplit) {
     float s = 0.f;
@@ -494,24 +488,43 @@ __device__ inline float sum_splits(const float *__restrict__ part, long long tot
     return s;
 }
 
+// dW index of partial element e, or -1 for the padding of a partial tile
+__device__ inline long long partial_to_dw(long long e, const WGParams &p) {
+    const int lane = (int)(e & 63), v = (int)((e >> 6) & 3);
+    long long rest = e >> 8;
+    const int t = (int)(rest % p.RS);
+    rest /= p.RS;
+    const int mi = (int)(rest & 3), wave = (int)((rest >> 2) & 3);
+    const long long tile = rest >> 4;
+    const int kt = (int)(tile / p.ctiles), ct = (int)(tile % p.ctiles);
+    const int k = kt * WG_BK + mi * 16 + (lane >> 4) * 4 + v;
+    const int c = ct * WG_BC + wave * 16 + (lane & 15);
+    if (k >= p.K || c >= p.C) return -1;
+    return ((long long)k * p.C + c) * p.RS + (p.RS - 1 - t);
+}
+
 template <typename TOut>
-__global__ __launch_bounds__(256) void wgrad_sum(TOut *__restrict__ out, const float *__restrict__ part,
-                                                 long long total, int nsplit) {
+__global__ __launch_bounds__(256) void wgrad_sum(TOut *__restrict__ out, const float *__restrict__ part, WGParams p) {
+    const long long total = (long long)p.ktiles * p.ctiles * 4096 * p.RS;
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
          e += (long long)gridDim.x * blockDim.x) {
-        const float s = sum_splits(part, total, e, nsplit);
+        const long long o = partial_to_dw(e, p);
+        if (o < 0) continue;
+        const float s = sum_splits(part, total, e, p.nsplit);
         if constexpr (sizeof(TOut) == 4)
-            out[e] = s;
+            out[o] = s;
         else
-            out[e] = f32_to_bf16_bits(s);
+            out[o] = f32_to_bf16_bits(s);
     }
 }
-__global__ __launch_bounds__(256) void wgrad_sum_f16(_Float16 *__restrict__ out,
-                                                     const float *__restrict__ part, long long total,
-                                                     int nsplit) {
+__global__ __launch_bounds__(256) void wgrad_sum_f16(_Float16 *__restrict__ out, const float *__restrict__ part,
+                                                     WGParams p) {
+    const long long total = (long long)p.ktiles * p.ctiles * 4096 * p.RS;
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
-         e += (long long)gridDim.x * blockDim.x)
-        out[e] = (_Float16)sum_splits(part, total, e, nsplit);
+         e += (long long)gridDim.x * blockDim.x) {
+        const long long o = partial_to_dw(e, p);
+        if (o >= 0) out[o] = (_Float16)sum_splits(part, total, e, p.nsplit);
+    }
 }
 
 static size_t wg_lds(const WGParams &p) {
@@ -634,7 +647,7 @@ extern "C" long long afhip_k_conv2_nn_wgrad_rows_workspace(int in_type, int prec
     WGParams p;
     wg_fill(p, xd, wd, yd, st, pd, dl);
     if (!wg_plan(p, (int)wd[0], (int)st[0], (int)dl[0])) return 0;
-    return (long long)p.nsplit * p.K * p.C * p.RS * 4;
+    return (long long)p.nsplit * p.ktiles * p.ctiles * 4096 * p.RS * 4;
 }
 
 // what: 1 = partial tiles into the workspace, 2 = sum them into dw, 3 = both
@@ -669,15 +682,15 @@ static int rows_call(int what, int in_type, int out_type, int precision, void *d
         if (rc) return rc;
     }
     if (what & 2) {
-        const long long total = (long long)p.K * p.C * p.RS;
+        const long long total = (long long)p.ktiles * p.ctiles * 4096 * p.RS;
         long long blocks = (total + 255) / 256;
         if (blocks > kMaxGridBlocks) blocks = kMaxGridBlocks;
         if (out_type == DT_F32)
-            wgrad_sum<float><<<(unsigned)blocks, 256, 0, s>>>((float *)dw, part, total, p.nsplit);
+            wgrad_sum<float><<<(unsigned)blocks, 256, 0, s>>>((float *)dw, part, p);
         else if (out_type == DT_BF16 && in_type == DT_BF16)
-            wgrad_sum<uint16_t><<<(unsigned)blocks, 256, 0, s>>>((uint16_t *)dw, part, total, p.nsplit);
+            wgrad_sum<uint16_t><<<(unsigned)blocks, 256, 0, s>>>((uint16_t *)dw, part, p);
         else if (out_type == DT_F16 && in_type == DT_F16)
-            wgrad_sum_f16<<<(unsigned)blocks, 256, 0, s>>>((_Float16 *)dw, part, total, p.nsplit);
+            wgrad_sum_f16<<<(unsigned)blocks, 256, 0, s>>>((_Float16 *)dw, part, p);
         else
             return K_ERR_TYPE;
     }
