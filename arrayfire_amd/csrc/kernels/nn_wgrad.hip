@@ -19,6 +19,8 @@
 // into blocks of R_oh output rows of one image, staged in LDS; the blocks are
 // split across workgroups, whose FP32 partial tiles a second kernel sums in a
 // fixed order (deterministic).
+#include <type_traits>
+
 #include "kcommon.hpp"
 #include "../../../include/afhip_kernels.h"
 
@@ -468,26 +470,10 @@ __global__ __launch_bounds__(2 * WG_THREADS, 2) void wgrad_pc(float *__restrict_
             for (int v = 0; v < 4; ++v) dst[((mi * RS + t) * 4 + v) * 64] = acc[mi][t][v];
 }
 
-// Sums nsplit partial dW buffers and converts to the output type. The order
-// of the additions is fixed (groups of 8 splits, then the groups in order),
-// so results do not depend on scheduling. Eight partials are in flight per
-// lane. A split's partials are tiles (kt, ct) of [wave][mi][tap][v][lane]
-// floats (4096 * RS per tile): thread e reads element e of every split
-// (coalesced) and writes the dW element it maps to.
-__device__ inline float sum_splits(const float *__restrict__ part, long long total, long long e,
-                                   int nsplit) {
-    float s = 0.f;
-    int q = 0;
-    for (; q + 8 <= nsplit; q += 8) {
-        float v[8];
-#pragma unroll
-        for (int t = 0; t < 8; ++t) v[t] = part[(long long)(q + t) * total + e];
-        s += ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
-    }
-    for (; q < nsplit; ++q) s += part[(long long)q * total + e];
-    return s;
-}
-
+// Sums nsplit partial dW buffers and converts to the output type. A split's
+// partials are tiles (kt, ct) of [wave][mi][tap][v][lane] floats (4096 * RS
+// per tile); every partial element is read coalesced and written to the dW
+// element it maps to.
 // dW index of partial element e, or -1 for the padding of a partial tile
 __device__ inline long long partial_to_dw(long long e, const WGParams &p) {
     const int lane = (int)(e & 63), v = (int)((e >> 6) & 3);
@@ -503,27 +489,60 @@ __device__ inline long long partial_to_dw(long long e, const WGParams &p) {
     return ((long long)k * p.C + c) * p.RS + (p.RS - 1 - t);
 }
 
+// 1024 threads = 64 columns of 4 consecutive partial elements (16-byte loads)
+// x 16 split groups; group q sums splits [q*nsplit/16, (q+1)*nsplit/16) in
+// order with all of its loads in flight (up to 8 per batch), then the 16
+// group sums are added in a fixed pairwise tree: the order of the additions
+// never depends on scheduling.
+constexpr int WS_GROUPS = 16;
 template <typename TOut>
-__global__ __launch_bounds__(256) void wgrad_sum(TOut *__restrict__ out, const float *__restrict__ part, WGParams p) {
-    const long long total = (long long)p.ktiles * p.ctiles * 4096 * p.RS;
-    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
-         e += (long long)gridDim.x * blockDim.x) {
-        const long long o = partial_to_dw(e, p);
-        if (o < 0) continue;
-        const float s = sum_splits(part, total, e, p.nsplit);
-        if constexpr (sizeof(TOut) == 4)
-            out[o] = s;
-        else
-            out[o] = f32_to_bf16_bits(s);
-    }
-}
-__global__ __launch_bounds__(256) void wgrad_sum_f16(_Float16 *__restrict__ out, const float *__restrict__ part,
-                                                     WGParams p) {
-    const long long total = (long long)p.ktiles * p.ctiles * 4096 * p.RS;
-    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
-         e += (long long)gridDim.x * blockDim.x) {
-        const long long o = partial_to_dw(e, p);
-        if (o >= 0) out[o] = (_Float16)sum_splits(part, total, e, p.nsplit);
+__global__ __launch_bounds__(64 * WS_GROUPS) void wgrad_sum(TOut *__restrict__ out, const float *__restrict__ part,
+                                                            WGParams p) {
+    __shared__ float4 red[WS_GROUPS][64];
+    const long long total = (long long)p.ktiles * p.ctiles * 4096 * p.RS;  // a multiple of 256
+    const int col = threadIdx.x & 63, grp = threadIdx.x >> 6;
+    const int q0 = (int)((long long)grp * p.nsplit / WS_GROUPS);
+    const int q1 = (int)((long long)(grp + 1) * p.nsplit / WS_GROUPS);
+    for (long long e0 = blockIdx.x * 256LL; e0 < total; e0 += (long long)gridDim.x * 256) {
+        const long long e = e0 + col * 4;
+        float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int q = q0; q < q1; q += 8) {
+            float4 v[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+                v[t] = q + t < q1 ? *reinterpret_cast<const float4 *>(part + (long long)(q + t) * total + e)
+                                  : make_float4(0.f, 0.f, 0.f, 0.f);
+            s.x += ((v[0].x + v[1].x) + (v[2].x + v[3].x)) + ((v[4].x + v[5].x) + (v[6].x + v[7].x));
+            s.y += ((v[0].y + v[1].y) + (v[2].y + v[3].y)) + ((v[4].y + v[5].y) + (v[6].y + v[7].y));
+            s.z += ((v[0].z + v[1].z) + (v[2].z + v[3].z)) + ((v[4].z + v[5].z) + (v[6].z + v[7].z));
+            s.w += ((v[0].w + v[1].w) + (v[2].w + v[3].w)) + ((v[4].w + v[5].w) + (v[6].w + v[7].w));
+        }
+        red[grp][col] = s;
+        __syncthreads();
+#pragma unroll
+        for (int h = WS_GROUPS / 2; h >= 1; h >>= 1) {
+            if (grp < h) {
+                const float4 a = red[grp][col], b = red[grp + h][col];
+                red[grp][col] = make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+            }
+            __syncthreads();
+        }
+        if (grp == 0) {
+            const float4 t4 = red[0][col];
+            const float r[4] = {t4.x, t4.y, t4.z, t4.w};
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const long long o = partial_to_dw(e + i, p);
+                if (o < 0) continue;
+                if constexpr (sizeof(TOut) == 4)
+                    out[o] = r[i];
+                else if constexpr (std::is_same<TOut, _Float16>::value)
+                    out[o] = (_Float16)r[i];
+                else
+                    out[o] = f32_to_bf16_bits(r[i]);
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -683,14 +702,14 @@ static int rows_call(int what, int in_type, int out_type, int precision, void *d
     }
     if (what & 2) {
         const long long total = (long long)p.ktiles * p.ctiles * 4096 * p.RS;
-        long long blocks = (total + 255) / 256;
+        long long blocks = total / 256;
         if (blocks > kMaxGridBlocks) blocks = kMaxGridBlocks;
         if (out_type == DT_F32)
-            wgrad_sum<float><<<(unsigned)blocks, 256, 0, s>>>((float *)dw, part, p);
+            wgrad_sum<float><<<(unsigned)blocks, 64 * WS_GROUPS, 0, s>>>((float *)dw, part, p);
         else if (out_type == DT_BF16 && in_type == DT_BF16)
-            wgrad_sum<uint16_t><<<(unsigned)blocks, 256, 0, s>>>((uint16_t *)dw, part, p);
+            wgrad_sum<uint16_t><<<(unsigned)blocks, 64 * WS_GROUPS, 0, s>>>((uint16_t *)dw, part, p);
         else if (out_type == DT_F16 && in_type == DT_F16)
-            wgrad_sum_f16<<<(unsigned)blocks, 256, 0, s>>>((_Float16 *)dw, part, p);
+            wgrad_sum<_Float16><<<(unsigned)blocks, 64 * WS_GROUPS, 0, s>>>((_Float16 *)dw, part, p);
         else
             return K_ERR_TYPE;
     }
