@@ -36,7 +36,7 @@ PEAK = {"hbm_GBs": 8000.0, "bf16_TFs": 2500.0, "fp32_TFs": 157.3}
 KERNEL_SYMBOL = {
     "conv_fwd": "afhip::conv_pipe<1, unsigned short, unsigned short, 2, 3, 0, 3>",
     "conv_dgrad": "afhip::conv_pipe<1, unsigned short, unsigned short, 1, 3, 1, 3>",
-    "wgrad_partial": "afhip::wgrad_rows<3, 3, 1, 0, true>",
+    "wgrad_partial": "afhip::wgrad_pc<3, 3, 1, 0, true>",
     "wgrad_reduce": "afhip::wgrad_sum<unsigned short>",
 }
 # HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE (profiles/),
