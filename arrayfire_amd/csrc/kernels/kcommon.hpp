@@ -83,22 +83,25 @@ __device__ inline long long xcd_remap(long long bid, long long nblk) {
 // Loads an LH x LW halo tile of an FP32 image into LDS (row pitch LS) with
 // zeros outside [0,S0) x [0,S1). The tile origin (ix0, iy0) may be negative
 // or unaligned: rows are read as aligned float4s (when S0 % 4 == 0) that
-// cover the tile's columns, eight float4 slots per lane in flight before any
+// cover the tile's columns, NB float4 slots per lane in flight before any
 // LDS store, then scattered into place. NQ (float4 slots per row) must be at
 // least (LW + 6) / 4; a compile-time NQ keeps the slot decomposition cheap.
-template <int NT, int NQ>
-__device__ inline void load_halo_tile_f32(float *tile, const float *__restrict__ img, long long S0,
+// t: the calling thread's index among the NT threads that take part. Always
+// inlined: out of line, the LDS tile is written through generic pointers from
+// a called function with its own stack.
+template <int NT, int NQ, int NB = 8>
+__device__ __attribute__((always_inline)) inline void load_halo_tile_f32(float *tile, const float *__restrict__ img, long long S0,
                                           long long S1, long long ix0, long long iy0, int LH, int LW,
-                                          int LS) {
+                                          int LS, int t = (int)threadIdx.x) {
     const long long a0 = ix0 - (((ix0 % 4) + 4) % 4);  // floor to a multiple of 4
     const int off = (int)(ix0 - a0);                  // 0..3
     const int slots = LH * NQ;
     const bool vec = (S0 & 3) == 0;
-    for (int s0 = 0; s0 < slots; s0 += 8 * NT) {
-        float4 v[8];
+    for (int s0 = 0; s0 < slots; s0 += NB * NT) {
+        float4 v[NB];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int slot = s0 + u * NT + (int)threadIdx.x;
+        for (int u = 0; u < NB; ++u) {
+            const int slot = s0 + u * NT + t;
             v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
             const int r = slot / NQ, q = slot - (slot / NQ) * NQ;
             if (slot < slots && 4 * q - off < LW) {
@@ -118,8 +121,8 @@ __device__ inline void load_halo_tile_f32(float *tile, const float *__restrict__
             }
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int slot = s0 + u * NT + (int)threadIdx.x;
+        for (int u = 0; u < NB; ++u) {
+            const int slot = s0 + u * NT + t;
             const int r = slot / NQ, q = slot - (slot / NQ) * NQ;
             const int c0 = 4 * q - off;
             if (slot < slots && c0 < LW) {
