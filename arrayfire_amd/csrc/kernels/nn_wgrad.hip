@@ -19,6 +19,7 @@
 // into blocks of R_oh output rows of one image, staged in LDS; the blocks are
 // split across workgroups, whose FP32 partial tiles a second kernel sums in a
 // fixed order (deterministic).
+#include <cstdlib>
 #include <type_traits>
 
 #include "kcommon.hpp"
@@ -576,7 +577,10 @@ static bool wg_plan(WGParams &p, int fw, int s0, int d0) {
     p.dy_lpr_log2 = 0;
     while ((1 << p.dy_lpr_log2) < p.segrow) ++p.dy_lpr_log2;
     p.x_rpr = WG_THREADS / (p.PWpad / 8);
-    const bool pc_ok = p.dy_lpr_log2 <= 8 && p.x_rpr >= 1 && (long long)p.C * p.H * p.W * 2 < (1LL << 31) - 64 &&
+    // AFHIP_NN_WGRAD_PC=0 in the environment selects the single-buffer kernel
+    const char *pc_env = std::getenv("AFHIP_NN_WGRAD_PC");
+    const bool pc_ok = !(pc_env && pc_env[0] == '0') && p.dy_lpr_log2 <= 8 && p.x_rpr >= 1 &&
+                       (long long)p.C * p.H * p.W * 2 < (1LL << 31) - 64 &&
                        (long long)p.K * p.OH * p.OW * 2 < (1LL << 31) - 64;
     int r = 0;
     for (int pc = pc_ok ? 1 : 0; pc >= 0 && r < 1; --pc) {

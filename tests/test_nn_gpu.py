@@ -320,6 +320,22 @@ def test_filter_gradient_16bit_arrays(af, rng, g, dt, monkeypatch):
     assert oracle.rel_l2(dw.to_numpy(), dw2.to_numpy()) <= 2e-6
 
 
+@pytest.mark.parametrize("g", WGRAD_GEOMS, ids=_id)
+def test_filter_gradient_single_buffer_kernel(af, rng, g, monkeypatch):
+    """The single-buffer row-staged kernel (the fallback when two row blocks
+    do not fit in LDS) on the same geometries, BF16, against the FP64 closed
+    form of the rounded operands."""
+    monkeypatch.setenv("AFHIP_NN_WGRAD_PC", "0")
+    x, w = _data(rng, g)
+    X, Wt = af.Array(x).astype(af.bf16), af.Array(w).astype(af.bf16)
+    y = af.convolve2NN(X, Wt, g[2], g[3], g[4], out_type=af.f32)
+    dy = rng.standard_normal(y.dims).astype(np.float32)
+    DY = af.Array(dy).astype(af.bf16)
+    dw = af.convolve2GradientNN(DY, X, Wt, y, *g[2:], af.AF_CONV_GRADIENT_FILTER, out_type=af.f32)
+    ref = oracle.conv2_filter_grad_direct_f64(_rounded(DY), _rounded(X), w, *g[2:])
+    assert oracle.rel_l2(dw.to_numpy(), ref) <= 1e-5
+
+
 def _rounded(a):
     v = a.to_numpy()
     if v.dtype == np.uint16:
