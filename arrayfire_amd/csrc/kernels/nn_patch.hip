@@ -88,53 +88,34 @@ template <int MODE, typename TIn>
 __global__ __launch_bounds__(256) void pack_weights(void *__restrict__ out,
                                                     const TIn *__restrict__ w, int M, int CH,
                                                     int fw, int fh, int wC, int nchunks,
-                                                    int ktiles, int dgrad) {
-    // one lane per 16-byte output vector: VPK channels of one packed row
-    // (ktile, chunk, tap, m); consecutive lanes write consecutive vectors
+                                                    int dgrad) {
+    // one lane per packed element; blockIdx.x names the panel (ktile, chunk,
+    // tap), whose P_BM x KC elements blockIdx.y covers 256 at a time with the
+    // channel fastest, so a wave writes contiguous bytes
     constexpr int KC = PCfg<MODE>::KC;
-    constexpr int VPK = MODE == PM_F32 ? 4 : 8;
-    constexpr int NV = KC / VPK;  // vectors per row
+    constexpr int PANEL = P_BM * KC;
     const int RS = fw * fh;
-    const int rows = ktiles * nchunks * RS * P_BM;
-    const long long plane = (long long)rows * KC;
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < rows * NV; e += gridDim.x * blockDim.x) {
-        const int row = e / NV, vq = e - (e / NV) * NV;
-        const int m = row % P_BM;
-        int r = row / P_BM;
-        const int t = r % RS;
-        r /= RS;
-        const int cc = r % nchunks, kt = r / nchunks;
-        const int mm = kt * P_BM + m;
-        const int i = t % fw, j = t / fw;
-        // fwd: w_af[fw-1-i, fh-1-j, ch, mm] (KCRS memory, channel stride RS);
-        // dgrad: w_af[i, j, mm, ch] with mm the channel (filter stride wC*RS)
-        const long long base = !dgrad ? (long long)mm * wC * RS + (fh - 1 - j) * fw + (fw - 1 - i)
-                                      : (long long)mm * RS + j * fw + i;
-        const long long step = !dgrad ? RS : (long long)wC * RS;
-        float v[VPK];
-#pragma unroll
-        for (int c = 0; c < VPK; ++c) {
-            const int ch = cc * KC + vq * VPK + c;
-            v[c] = (mm < M && ch < CH) ? pld(w, base + ch * step) : 0.f;
-        }
-        const long long e0 = (long long)row * KC + vq * VPK;
-        if constexpr (MODE == PM_F32) {
-            *reinterpret_cast<float4 *>(reinterpret_cast<float *>(out) + e0) = make_float4(v[0], v[1], v[2], v[3]);
-        } else {
-            uint16_t h[8], l[8];
-#pragma unroll
-            for (int c = 0; c < 8; ++c) {
-                h[c] = MODE == PM_F16 ? pf32_to_f16_bits(v[c]) : f32_to_bf16_bits(v[c]);
-                l[c] = MODE == PM_SPLIT ? f32_to_bf16_bits(v[c] - bf16_bits_to_f32(h[c])) : 0;
-            }
-            *reinterpret_cast<uint4 *>(reinterpret_cast<uint16_t *>(out) + e0) =
-                make_uint4(h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16),
-                           h[4] | ((uint32_t)h[5] << 16), h[6] | ((uint32_t)h[7] << 16));
-            if constexpr (MODE == PM_SPLIT)
-                *reinterpret_cast<uint4 *>(reinterpret_cast<uint16_t *>(out) + plane + e0) =
-                    make_uint4(l[0] | ((uint32_t)l[1] << 16), l[2] | ((uint32_t)l[3] << 16),
-                               l[4] | ((uint32_t)l[5] << 16), l[6] | ((uint32_t)l[7] << 16));
-        }
+    const int pnl = blockIdx.x;
+    const int t = pnl % RS, q = pnl / RS;
+    const int cc = q % nchunks, kt = q / nchunks;
+    const int i = t % fw, j = t / fw;
+    const int idx = blockIdx.y * 256 + threadIdx.x;
+    const int m = idx / KC, kc = idx % KC;
+    const int mm = kt * P_BM + m, ch = cc * KC + kc;
+    // fwd: w_af[fw-1-i, fh-1-j, ch, mm] (KCRS memory, channel stride RS);
+    // dgrad: w_af[i, j, mm, ch] with mm the channel (filter stride wC*RS)
+    const long long src = !dgrad ? ((long long)mm * wC + ch) * RS + (fh - 1 - j) * fw + (fw - 1 - i)
+                                 : ((long long)ch * wC + mm) * RS + j * fw + i;
+    const float v = (mm < M && ch < CH) ? pld(w, src) : 0.f;
+    const long long e = (long long)pnl * PANEL + idx;
+    if constexpr (MODE == PM_F32) {
+        reinterpret_cast<float *>(out)[e] = v;
+    } else {
+        const uint16_t h = MODE == PM_F16 ? pf32_to_f16_bits(v) : f32_to_bf16_bits(v);
+        reinterpret_cast<uint16_t *>(out)[e] = h;
+        if constexpr (MODE == PM_SPLIT)
+            reinterpret_cast<uint16_t *>(out)[(long long)gridDim.x * PANEL + e] =
+                f32_to_bf16_bits(v - bf16_bits_to_f32(h));
     }
 }
 
@@ -1066,16 +1047,14 @@ static int launch(void *y, const void *x, const void *w, PatchParams &p, int dgr
                   int what, hipStream_t s) {
     using Cfg = PCfg<MODE>;
     if (what & 1) {
-        const long long rows = (long long)p.ktiles * p.nchunks * p.RS * P_BM;
         // weights: K x C x fh x fw of the original filter; for dgrad the
         // GEMM's M is the original C and its channels are the original K
         const int wC = dgrad ? p.K : p.C;
-        const long long vecs = rows * (Cfg::KC / (MODE == PM_F32 ? 4 : 8));
-        if (vecs >= (1LL << 31)) return K_ERR_SIZE;
-        long long blocks = (vecs + 255) / 256;
-        if (blocks > kMaxGridBlocks) blocks = kMaxGridBlocks;
-        pack_weights<MODE, TIn><<<(unsigned)blocks, 256, 0, s>>>(ws, (const TIn *)w, p.K, p.C, p.fw,
-                                                                   p.fh, wC, p.nchunks, p.ktiles, dgrad);
+        const long long panels = (long long)p.ktiles * p.nchunks * p.RS;
+        if (panels * P_BM * Cfg::KC >= (1LL << 31)) return K_ERR_SIZE;
+        const dim3 grid((unsigned)panels, P_BM * Cfg::KC / 256);
+        pack_weights<MODE, TIn><<<grid, 256, 0, s>>>(ws, (const TIn *)w, p.K, p.C, p.fw, p.fh, wC,
+                                                      p.nchunks, dgrad);
     }
     if (what & 2) {
         PatchParams q;
