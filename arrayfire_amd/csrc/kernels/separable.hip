@@ -13,6 +13,8 @@
 //    second LDS tile of 64 x (64+F-1) (rounded to float, the reference's temp
 //    precision), then the row pass to HBM: one read and one write of every
 //    element.
+#include <stdlib.h>
+
 #include "kcommon.hpp"
 #include "../../../include/afhip_kernels.h"
 
@@ -174,6 +176,168 @@ __global__ __launch_bounds__(256) void sep_fused_f32(float *__restrict__ out,
     }
 }
 
+// Strip form of the fused FP32 kernel, for inputs whose rows are whole
+// 16-byte vectors (S0 % 4 == 0, 16-byte aligned base, images below 2 GiB).
+// A workgroup owns a 64-column output strip over SS_SEG output rows of one
+// image and walks down it in steps of 16 rows:
+//  * the column pass of step u turns the 16 input rows iy0 + 16u .. +15 (in
+//    LDS, xin) into 16 temp rows of a ring in LDS, so the rf - 1 halo rows
+//    are computed once per strip segment, not once per output tile; each temp
+//    row is written twice, RING rows apart, so that a row-pass read sequence
+//    never wraps;
+//  * the row pass of output rows 16(u - P) .. +15 reads the ring once the
+//    rows it needs are there (P = ceil((F - 1) / 16) steps behind);
+//  * the input rows of step u + 2 are loaded into registers while step u
+//    runs (buffer loads: rows and 16-byte column groups outside the image
+//    read zeros) and written to xin between the two passes of step u + 1.
+// Sums run in the same order as sep_fused_f32 (flipped taps ascending); taps
+// past the filter length are skipped by uniform branches (EXACT: cf == rf ==
+// F, no branches), never multiplied by 0. The xin origin is the strip's first input column
+// floored to a multiple of 4; the column pass reads its window at the
+// remaining offset.
+constexpr int SS_SEG = 256;  // output rows per workgroup
+
+struct SepStripParams {
+    dim_t S0, S1, O0, O1;
+    dim_t nbx, nseg, nblk;
+    int cf, rf, sx, sy;
+};
+
+template <int F, bool EXACT>
+__global__ __launch_bounds__(256) void sep_strip_f32(float *__restrict__ out,
+                                                     const float *__restrict__ in,
+                                                     const float *__restrict__ cfilt,
+                                                     const float *__restrict__ rfilt,
+                                                     SepStripParams p) {
+    constexpr int H = F - 1;
+    constexpr int P = (H + 15) / 16;                    // column-only steps before the first row pass
+    constexpr int RING = 16 * (P + 1) <= 32 ? 32 : 64;  // temp rows kept (power of two)
+    constexpr int XQ = (64 + H + 3 + 3) / 4;            // float4s loaded per input row
+    constexpr int XP = 4 * XQ;                          // xin row pitch (floats)
+    constexpr int NSLOT = (16 * XQ + 255) / 256;        // 16-byte loads per lane per step
+    constexpr int kOOB = 0x7fffffc0;
+    __shared__ __attribute__((aligned(16))) float xin[16 * XP];
+    __shared__ __attribute__((aligned(16))) float ring[2 * RING * 64];
+
+    const dim_t blk = xcd_remap(blockIdx.x, p.nblk);
+    const int bx = (int)(blk % p.nbx);
+    const dim_t t = blk / p.nbx;
+    const int sg = (int)(t % p.nseg);
+    const dim_t b = t / p.nseg;
+    const float *img = in + b * p.S0 * p.S1;
+    out += b * p.O0 * p.O1;
+
+    const int S0 = (int)p.S0, S1 = (int)p.S1, O0 = (int)p.O0, O1 = (int)p.O1;
+    const int cf = p.cf, rf = p.rf;
+    const int ox0 = bx * 64, oy0 = sg * SS_SEG;
+    const int cx = ox0 + p.sx - (cf - 1);             // first input column the strip reads
+    const int a0 = cx - (((cx % 4) + 4) % 4);         // floored to a multiple of 4
+    const int off = cx - a0;                          // 0..3, the same for every strip
+    const int iy0 = oy0 + p.sy - (rf - 1);
+    const int nsteps = ((O1 - oy0 < SS_SEG ? O1 - oy0 : SS_SEG) + 15) / 16;
+    const int U = nsteps + P;                         // column-pass steps
+
+    float wc[F], wr[F];  // flipped taps (only k < cf, k < rf are read)
+#pragma unroll
+    for (int k = 0; k < F; ++k) {
+        wc[k] = cfilt[k < cf ? cf - 1 - k : 0];
+        wr[k] = rfilt[k < rf ? rf - 1 - k : 0];
+    }
+
+    // this lane's load slots: row s_row, byte offset s_col in the image row
+    const auto rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(img), (short)0, S0 * S1 * 4,
+                                                        0x00020000);
+    int s_row[NSLOT], s_col[NSLOT], s_lds[NSLOT];
+#pragma unroll
+    for (int s = 0; s < NSLOT; ++s) {
+        const int slot = s * 256 + (int)threadIdx.x;
+        const int r = slot / XQ, q = slot - (slot / XQ) * XQ;
+        const int gx = a0 + 4 * q;
+        const bool ok = slot < 16 * XQ && gx >= 0 && gx < S0;  // a float4 is wholly in or out
+        s_row[s] = ok ? r : -(1 << 28);                      // a dead slot's row is never in range
+        s_col[s] = ok ? gx * 4 : 0;
+        s_lds[s] = slot < 16 * XQ ? r * XP + 4 * q : -1;
+    }
+    float4 pre[NSLOT];
+    auto issue = [&](int u) __attribute__((always_inline)) {
+#pragma unroll
+        for (int s = 0; s < NSLOT; ++s) {
+            const int gy = iy0 + 16 * u + s_row[s];
+            const int o = (unsigned)gy < (unsigned)S1 ? gy * (S0 * 4) + s_col[s] : kOOB;
+            pre[s] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rsrc, o, 0, 0));
+        }
+    };
+    auto commit = [&]() __attribute__((always_inline)) {
+#pragma unroll
+        for (int s = 0; s < NSLOT; ++s)
+            if (s_lds[s] >= 0) *reinterpret_cast<float4 *>(xin + s_lds[s]) = pre[s];
+    };
+
+    issue(0);
+    commit();
+    if (U > 1) issue(1);
+    __syncthreads();
+
+    const int cr = threadIdx.x >> 4, cx0 = 4 * (threadIdx.x & 15);  // column pass: row, 4 outputs
+    const int rx = threadIdx.x & 63, ry0 = 4 * (threadIdx.x >> 6);  // row pass: column, 4 outputs
+    const float *wsrc = xin + cr * XP + cx0 + off;
+    for (int u = 0; u < U; ++u) {
+        {   // column pass: temp row 16u + cr, outputs cx0 .. cx0 + 3
+            float win[4 + H];
+#pragma unroll
+            for (int q = 0; q < 4 + H; ++q) win[q] = wsrc[q];
+            float acc[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int k = 0; k < F; ++k)
+                if (EXACT || k < cf) {  // uniform
+                    if constexpr (!EXACT) asm volatile("");  // keeps the branch (no select of products)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) acc[e] = __builtin_fmaf(win[e + k], wc[k], acc[e]);
+                }
+            const float4 v = make_float4(acc[0], acc[1], acc[2], acc[3]);
+            float *dst = ring + ((16 * u + cr) & (RING - 1)) * 64 + cx0;
+            *reinterpret_cast<float4 *>(dst) = v;
+            *reinterpret_cast<float4 *>(dst + RING * 64) = v;
+        }
+        __syncthreads();
+        if (u + 1 < U) commit();
+        if (u + 2 < U) issue(u + 2);
+        if (u >= P) {  // row pass: output rows 16(u - P) + ry0 .. + 3 of column rx
+            const int y0 = 16 * (u - P) + ry0;
+            const float *src = ring + (((16 * (u - P)) & (RING - 1)) + ry0) * 64 + rx;
+            float v[4 + H];
+#pragma unroll
+            for (int q = 0; q < 4 + H; ++q) v[q] = src[q * 64];
+            float acc[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int k = 0; k < F; ++k)
+                if (EXACT || k < rf) {  // uniform
+                    if constexpr (!EXACT) asm volatile("");
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) acc[i] = __builtin_fmaf(v[i + k], wr[k], acc[i]);
+                }
+            const int gx = ox0 + rx;
+            float *o = out + (dim_t)(oy0 + y0) * O0 + gx;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (gx < O0 && oy0 + y0 + i < O1) o[(dim_t)i * O0] = acc[i];
+        }
+        __syncthreads();
+    }
+}
+
+// The strip kernel takes images whose rows are whole 16-byte vectors from a
+// 16-byte aligned base, below 2 GiB each (its buffer offsets are 32-bit).
+// AFHIP_SEP_STRIP=0 in the environment selects the tile kernel instead.
+static bool sep_strip_ok(const long long *sdims, const void *in) {
+    static const bool enabled = [] {
+        const char *e = getenv("AFHIP_SEP_STRIP");
+        return !(e && e[0] == '0');
+    }();
+    return enabled && sdims[0] % 4 == 0 && ((uintptr_t)in & 15) == 0 &&
+           sdims[0] * sdims[1] * 4 < 0x7fffffc0LL;
+}
+
 static unsigned grid_for(dim_t n) {
     dim_t b = (n + 255) / 256;
     if (b > kMaxGridBlocks) b = kMaxGridBlocks;
@@ -218,6 +382,44 @@ extern "C" int afhip_k_convolve2_sep(int dtype, void *out, const long long *odim
     if (expand) p.td[0] += cflen - 1;
     p.expand = expand;
     p.flen = 0;
+    if (dtype == DT_F32 && tmp == nullptr && cflen <= SF_MAXF && rflen <= SF_MAXF &&
+        sep_strip_ok(sdims, in)) {
+        SepStripParams f;
+        f.S0 = sdims[0];
+        f.S1 = sdims[1];
+        f.O0 = odims[0];
+        f.O1 = odims[1];
+        f.nbx = (f.O0 + 63) / 64;
+        f.nseg = (f.O1 + SS_SEG - 1) / SS_SEG;
+        f.nblk = f.nbx * f.nseg * odims[2] * odims[3];
+        f.cf = (int)cflen;
+        f.rf = (int)rflen;
+        f.sx = expand ? 0 : (int)(cflen >> 1);
+        f.sy = expand ? 0 : (int)(rflen >> 1);
+        if (f.nblk == 0) return K_OK;
+        if (f.nblk > 0x7fffffffLL) return K_ERR_SIZE;
+        const int fm = (int)(cflen > rflen ? cflen : rflen);
+#define SEPS(FB)                                                                                           \
+    do {                                                                                                   \
+        if (cflen == FB && rflen == FB)                                                                    \
+            sep_strip_f32<FB, true><<<dim3((unsigned)f.nblk), 256, 0, s>>>(                                \
+                (float *)out, (const float *)in, (const float *)cfilt, (const float *)rfilt, f);           \
+        else                                                                                               \
+            sep_strip_f32<FB, false><<<dim3((unsigned)f.nblk), 256, 0, s>>>(                               \
+                (float *)out, (const float *)in, (const float *)cfilt, (const float *)rfilt, f);           \
+    } while (0)
+        if (fm <= 3) SEPS(3);
+        else if (fm <= 5) SEPS(5);
+        else if (fm <= 7) SEPS(7);
+        else if (fm <= 9) SEPS(9);
+        else if (fm <= 11) SEPS(11);
+        else if (fm <= 13) SEPS(13);
+        else if (fm <= 15) SEPS(15);
+        else if (fm <= 17) SEPS(17);
+        else SEPS(SF_MAXF);
+#undef SEPS
+        return launch_status();
+    }
     if (dtype == DT_F32 && tmp == nullptr && cflen <= SF_MAXF && rflen <= SF_MAXF) {
         SepFusedParams f;
         f.S0 = sdims[0];

@@ -28,6 +28,16 @@ CASES = [
     # not established, so the case stays off the GPU.
     pytest.param(((70, 50), 19, 21), marks=pytest.mark.skip(reason="GPU fault under investigation")),
     ((64, 65), 25, 23),         # 33-tap kernel with both filters shorter
+    # rows of whole 16-byte vectors take the strip kernel: several strips
+    # (the last partial), several 256-row segments (the last partial), even
+    # filters (column origin not a multiple of 4), every tap bucket
+    ((96, 600), 7, 3),
+    ((100, 530, 2), 17, 17),
+    ((36, 520, 3), 33, 33),
+    ((200, 260), 2, 4),
+    ((68, 257), 9, 6),
+    ((132, 300), 13, 31),
+    ((1024, 64), 16, 17),
 ]
 
 
