@@ -80,6 +80,18 @@ __device__ inline long long xcd_remap(long long bid, long long nblk) {
     return xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
 }
 
+// Output rows per workgroup of a strip kernel (a multiple of 16, at most
+// 256): long segments amortise the halo rows each segment loads and
+// computes again, short ones keep at least about 2048 workgroups (8 per CU)
+// when the strips alone are too few.
+inline int strip_rows(long long rows, long long strips) {
+    const long long steps = (rows + 15) / 16;
+    long long per = steps * strips / 2048;
+    if (per < 1) per = 1;
+    if (per > 16) per = 16;
+    return (int)(16 * per);
+}
+
 // Loads an LH x LW halo tile of an FP32 image into LDS (row pitch LS) with
 // zeros outside [0,S0) x [0,S1). The tile origin (ix0, iy0) may be negative
 // or unaligned: rows are read as aligned float4s (when S0 % 4 == 0) that

@@ -178,7 +178,7 @@ __global__ __launch_bounds__(256) void sep_fused_f32(float *__restrict__ out,
 
 // Strip form of the fused FP32 kernel, for inputs whose rows are whole
 // 16-byte vectors (S0 % 4 == 0, 16-byte aligned base, images below 2 GiB).
-// A workgroup owns a 64-column output strip over SS_SEG output rows of one
+// A workgroup owns a 64-column output strip over p.seg output rows of one
 // image and walks down it in steps of 16 rows:
 //  * the column pass of step u turns the 16 input rows iy0 + 16u .. +15 (in
 //    LDS, xin) into 16 temp rows of a ring in LDS, so the rf - 1 halo rows
@@ -195,12 +195,11 @@ __global__ __launch_bounds__(256) void sep_fused_f32(float *__restrict__ out,
 // F, no branches), never multiplied by 0. The xin origin is the strip's first input column
 // floored to a multiple of 4; the column pass reads its window at the
 // remaining offset.
-constexpr int SS_SEG = 256;  // output rows per workgroup
-
 struct SepStripParams {
     dim_t S0, S1, O0, O1;
     dim_t nbx, nseg, nblk;
     int cf, rf, sx, sy;
+    int seg;  // output rows per workgroup (a multiple of 16)
 };
 
 template <int F, bool EXACT>
@@ -229,12 +228,12 @@ __global__ __launch_bounds__(256) void sep_strip_f32(float *__restrict__ out,
 
     const int S0 = (int)p.S0, S1 = (int)p.S1, O0 = (int)p.O0, O1 = (int)p.O1;
     const int cf = p.cf, rf = p.rf;
-    const int ox0 = bx * 64, oy0 = sg * SS_SEG;
+    const int ox0 = bx * 64, oy0 = sg * p.seg;
     const int cx = ox0 + p.sx - (cf - 1);             // first input column the strip reads
     const int a0 = cx - (((cx % 4) + 4) % 4);         // floored to a multiple of 4
     const int off = cx - a0;                          // 0..3, the same for every strip
     const int iy0 = oy0 + p.sy - (rf - 1);
-    const int nsteps = ((O1 - oy0 < SS_SEG ? O1 - oy0 : SS_SEG) + 15) / 16;
+    const int nsteps = ((O1 - oy0 < p.seg ? O1 - oy0 : p.seg) + 15) / 16;
     const int U = nsteps + P;                         // column-pass steps
 
     float wc[F], wr[F];  // flipped taps (only k < cf, k < rf are read)
@@ -390,7 +389,8 @@ extern "C" int afhip_k_convolve2_sep(int dtype, void *out, const long long *odim
         f.O0 = odims[0];
         f.O1 = odims[1];
         f.nbx = (f.O0 + 63) / 64;
-        f.nseg = (f.O1 + SS_SEG - 1) / SS_SEG;
+        f.seg = strip_rows(f.O1, f.nbx * odims[2] * odims[3]);
+        f.nseg = (f.O1 + f.seg - 1) / f.seg;
         f.nblk = f.nbx * f.nseg * odims[2] * odims[3];
         f.cf = (int)cflen;
         f.rf = (int)rflen;

@@ -14,6 +14,8 @@
 //    lives in SGPRs (uniform scalar loads).
 //  * conv_generic: any rank, dtype, filter size and batch kind; one lane per
 //    output, grid-stride loop. Used where no tiled variant applies.
+#include <stdlib.h>
+
 #include "kcommon.hpp"
 #include "../../../include/afhip_kernels.h"
 
@@ -97,6 +99,7 @@ struct Tile2Params {
     dim_t batch2;           // extent of batch dim 2 (dim 3 is the remainder)
     dim_t nbx, nby, nblk;   // tiles along dim0, dim1, total workgroups
     int sx, sy;             // centring offsets (0 for EXPAND)
+    int seg;                // conv2_strip: output rows per workgroup (a multiple of 16)
 };
 
 constexpr int T2_TX = 4, T2_TY = 8, T2_NTX = 32, T2_NTY = 8;
@@ -182,6 +185,157 @@ __global__ __launch_bounds__(256) void conv2_tile(float *__restrict__ out,
             for (int i = 0; i < TX; ++i)
                 if (oxt + i < p.O0) orow[oxt + i] = acc[j][i];
         }
+    }
+}
+
+// Strip form of conv2_tile for inputs whose rows are whole 16-byte vectors
+// (S0 % 4 == 0, 16-byte aligned slices, slices below 2 GiB). A workgroup owns
+// a 128-column output strip over p.seg output rows of one slice and walks
+// down it in steps of 16 output rows. Input rows live in a ring of 4 blocks
+// of 16 rows in LDS; output step s reads blocks s and s + 1 (FY <= 17):
+//  * at the start of step s, block s + 2 (loaded into registers during step
+//    s - 1) is written to its ring slot, whose previous block s - 2 no step
+//    after s - 1 reads, and the loads of block s + 3 are issued;
+//  * the taps of step s then run from LDS; one barrier ends the step.
+// Rows and 16-byte column groups outside the slice read zeros (buffer loads).
+// Ring column c (input column ix0 + c) sits at row offset 4 + c: a loaded
+// float4 lands whole at 4 + 4q - off, inside the row's margins, so no column
+// test is needed. Each lane computes 4 columns x 2 rows with the flipped
+// filter in scalar registers, summing in the same order as conv2_tile.
+template <int FX, int FY>
+__global__ __launch_bounds__(256) void conv2_strip(float *__restrict__ out,
+                                                   const float *__restrict__ in,
+                                                   const float *__restrict__ filt,
+                                                   Tile2Params p) {
+    constexpr int BX = 128, TX = 4, TY = 2;
+    constexpr int LW = BX + FX - 1;                    // input columns a strip reads
+    constexpr int XQ = (LW + 3 + 3) / 4;               // float4s loaded per input row
+    constexpr int LS = 4 + 4 * XQ;                     // ring row pitch (floats)
+    constexpr int NSLOT = (16 * XQ + 255) / 256;       // 16-byte loads per lane per block
+    constexpr int NV = (TX + FX - 1 + 3) / 4;          // float4 reads per row segment
+    constexpr int kOOB = 0x7fffffc0;
+    static_assert(FY <= 17, "two ring blocks per output step");
+    static_assert(4 + 4 * 31 + 4 * NV <= LS, "row segment reads stay in the row");
+    __shared__ __attribute__((aligned(16))) float ring[64 * LS];
+
+    const dim_t blk = xcd_remap(blockIdx.x, p.nblk);
+    const int bx = (int)(blk % p.nbx);
+    const dim_t t = blk / p.nbx;
+    const int sg = (int)(t % p.nby);
+    const dim_t b = t / p.nby;
+    const dim_t b2 = b % p.batch2, b3 = b / p.batch2;
+    const float *img = in + b2 * p.in_step2 + b3 * p.in_step3;
+    out += b2 * p.out_step2 + b3 * p.out_step3;
+    filt += b2 * p.filt_step2 + b3 * p.filt_step3;
+
+    float wf[FY][FX];
+#pragma unroll
+    for (int ky = 0; ky < FY; ++ky)
+#pragma unroll
+        for (int kx = 0; kx < FX; ++kx) wf[ky][kx] = filt[(FY - 1 - ky) * FX + (FX - 1 - kx)];
+
+    const int S0 = (int)p.S0, S1 = (int)p.S1, O0 = (int)p.O0, O1 = (int)p.O1;
+    const int ox0 = bx * BX, oy0 = sg * p.seg;
+    const int ix0 = ox0 + p.sx - (FX - 1), iy0 = oy0 + p.sy - (FY - 1);
+    const int a0 = ix0 - (((ix0 % 4) + 4) % 4);        // ix0 floored to a multiple of 4
+    const int off = ix0 - a0;                          // 0..3
+    const int nsteps = ((O1 - oy0 < p.seg ? O1 - oy0 : p.seg) + 15) / 16;
+    const int nblocks = nsteps + 1;                    // input blocks the segment reads
+
+    const auto rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(img), (short)0, S0 * S1 * 4,
+                                                        0x00020000);
+    int s_row[NSLOT], s_col[NSLOT], s_lds[NSLOT];
+#pragma unroll
+    for (int s = 0; s < NSLOT; ++s) {
+        const int slot = s * 256 + (int)threadIdx.x;
+        const int r = slot / XQ, q = slot - (slot / XQ) * XQ;
+        const int gx = a0 + 4 * q;
+        const bool ok = slot < 16 * XQ && gx >= 0 && gx < S0;  // a float4 is wholly in or out
+        s_row[s] = ok ? r : -(1 << 28);                      // a dead slot's row is never in range
+        s_col[s] = ok ? gx * 4 : 0;
+        s_lds[s] = slot < 16 * XQ ? r * LS + 4 + 4 * q - off : -1;
+    }
+    float4 pre[NSLOT], pre1[NSLOT];
+    auto issue = [&](float4 (&v)[NSLOT], int bl) __attribute__((always_inline)) {
+#pragma unroll
+        for (int s = 0; s < NSLOT; ++s) {
+            const int gy = iy0 + 16 * bl + s_row[s];
+            const int o = (unsigned)gy < (unsigned)S1 ? gy * (S0 * 4) + s_col[s] : kOOB;
+            v[s] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rsrc, o, 0, 0));
+        }
+    };
+    auto commit = [&](const float4 (&v)[NSLOT], int bl) __attribute__((always_inline)) {
+        float *dst = ring + (bl & 3) * (16 * LS);
+#pragma unroll
+        for (int s = 0; s < NSLOT; ++s)
+            if (s_lds[s] >= 0) {
+                float *d = dst + s_lds[s];
+                d[0] = v[s].x;
+                d[1] = v[s].y;
+                d[2] = v[s].z;
+                d[3] = v[s].w;
+            }
+    };
+
+    issue(pre, 0);
+    issue(pre1, 1);
+    commit(pre, 0);
+    commit(pre1, 1);
+    if (nblocks > 2) issue(pre, 2);
+    __syncthreads();
+
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const int oxt = ox0 + tx * TX;
+    const bool vec_out = (O0 & 3) == 0 && oxt + TX <= O0;
+    for (int st = 0; st < nsteps; ++st) {
+        if (st + 2 < nblocks) commit(pre, st + 2);
+        if (st + 3 < nblocks) issue(pre, st + 3);
+        float acc[TY][TX];
+#pragma unroll
+        for (int j = 0; j < TY; ++j)
+#pragma unroll
+            for (int i = 0; i < TX; ++i) acc[j][i] = 0.f;
+        const int rb = 16 * st + ty * TY;              // first input row (segment-relative)
+#pragma unroll
+        for (int r = 0; r < TY + FY - 1; ++r) {
+            const float *row = ring + ((rb + r) & 63) * LS + 4 + tx * TX;
+            float v[NV * 4];
+#pragma unroll
+            for (int q = 0; q < NV; ++q) {
+                const float4 f4 = *reinterpret_cast<const float4 *>(row + 4 * q);
+                v[4 * q + 0] = f4.x;
+                v[4 * q + 1] = f4.y;
+                v[4 * q + 2] = f4.z;
+                v[4 * q + 3] = f4.w;
+            }
+#pragma unroll
+            for (int j = 0; j < TY; ++j) {
+                const int ky = r - j;
+                if (ky >= 0 && ky < FY) {
+#pragma unroll
+                    for (int i = 0; i < TX; ++i)
+#pragma unroll
+                        for (int kx = 0; kx < FX; ++kx)
+                            acc[j][i] = __builtin_fmaf(v[i + kx], wf[ky][kx], acc[j][i]);
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < TY; ++j) {
+            const int oy = oy0 + rb + j;
+            if (oy < O1) {
+                float *orow = out + (dim_t)oy * O0;
+                if (vec_out) {
+                    *reinterpret_cast<float4 *>(orow + oxt) =
+                        make_float4(acc[j][0], acc[j][1], acc[j][2], acc[j][3]);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < TX; ++i)
+                        if (oxt + i < O0) orow[oxt + i] = acc[j][i];
+                }
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -280,6 +434,30 @@ static int launch_generic(void *out, const void *in, const void *filt,
     return launch_status();
 }
 
+// Odd square filters up to 9x9 have a strip variant (conv2_strip).
+static Tile2Fn strip2_lookup(dim_t fx, dim_t fy) {
+    if (fx != fy) return nullptr;
+    switch (fx) {
+        case 3: return conv2_strip<3, 3>;
+        case 5: return conv2_strip<5, 5>;
+        case 7: return conv2_strip<7, 7>;
+        case 9: return conv2_strip<9, 9>;
+        default: return nullptr;
+    }
+}
+
+// conv2_strip takes slices whose rows are whole 16-byte vectors from 16-byte
+// aligned addresses, below 2 GiB each (its buffer offsets are 32-bit).
+// AFHIP_CONV2_STRIP=0 in the environment selects conv2_tile instead.
+static bool strip2_ok(const SpatialParams &p, const void *in) {
+    static const bool enabled = [] {
+        const char *e = getenv("AFHIP_CONV2_STRIP");
+        return !(e && e[0] == '0');
+    }();
+    return enabled && p.sd[0] % 4 == 0 && p.in_step[2] % 4 == 0 && p.in_step[3] % 4 == 0 &&
+           ((uintptr_t)in & 15) == 0 && p.sd[0] * p.sd[1] * 4 < 0x7fffffc0LL;
+}
+
 static int launch_tile2(float *out, const float *in, const float *filt,
                         const SpatialParams &p, Tile2Fn fn, hipStream_t s) {
     Tile2Params t;
@@ -294,8 +472,11 @@ static int launch_tile2(float *out, const float *in, const float *filt,
     t.filt_step2 = p.filt_step[2];
     t.filt_step3 = p.filt_step[3];
     t.batch2 = p.batch[2];
-    t.nbx = (t.O0 + T2_BX - 1) / T2_BX;
-    t.nby = (t.O1 + T2_BY - 1) / T2_BY;
+    Tile2Fn strip = strip2_ok(p, in) ? strip2_lookup(p.fd[0], p.fd[1]) : nullptr;
+    if (strip) fn = strip;
+    t.nbx = (t.O0 + T2_BX - 1) / T2_BX;  // both kernels are 128 columns wide
+    t.seg = strip ? strip_rows(t.O1, t.nbx * p.batch[2] * p.batch[3]) : 0;
+    t.nby = strip ? (t.O1 + t.seg - 1) / t.seg : (t.O1 + T2_BY - 1) / T2_BY;
     const dim_t nb = p.batch[2] * p.batch[3];
     t.nblk = t.nbx * t.nby * nb;
     t.sx = p.expand ? 0 : (int)(p.fd[0] / 2);

@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 import oracle
+from slab import slab_check
 
 pytestmark = pytest.mark.gpu
 
@@ -117,3 +118,18 @@ def test_padded_taps_never_read_the_image(af, mode):
     fin = np.isfinite(ref)
     assert np.array_equal(np.isfinite(got), fin)
     np.testing.assert_allclose(got[fin], ref[fin], rtol=1e-5, atol=1e-5)
+
+
+def test_strip_long_segments(af, rng):
+    """32 slices of 64 x 4096 give the strip kernel 64-row segments, so its
+    temp-row ring wraps within a workgroup; checked on row slabs at segment
+    edges and at both image borders."""
+    x = rng.random((64, 4096, 32), dtype=np.float32)
+    cf = rng.uniform(-1, 1, 17).astype(np.float32)
+    rf = rng.uniform(-1, 1, 17).astype(np.float32)
+    for mode in (0, 1):
+        got = af.convolve_sep(cf, rf, x, mode).to_numpy()
+        h = 0 if mode else 8
+        for k in (0, 13, 31):
+            for y0 in (0, 48, 64, 120, 2040, got.shape[1] - 16):
+                slab_check(got, x, k, lambda xs: oracle.convolve2_sep(cf, rf, xs, True), 17, h, h, y0)

@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 import oracle
+from slab import slab_check
 
 pytestmark = pytest.mark.gpu
 
@@ -31,6 +32,14 @@ CASES = [
     ((33, 31), (2, 4), 2),
     ((40, 30), (11, 3), 2),           # generic 2-D
     ((130, 70), (1, 1), 2),
+    # rows of whole 16-byte vectors with odd square filters take the strip
+    # kernel: partial last strip, several 256-row segments, every batch kind
+    ((260, 530, 2), (7, 7), 2),
+    ((132, 300), (9, 9), 2),
+    ((68, 40), (5, 5, 3), 2),         # RHS
+    ((36, 20, 2, 2), (3, 3, 2, 2), 2),  # SAME
+    ((1024, 24), (7, 7), 2),
+    ((64, 600), (5, 5), 2),
     ((10, 11, 2, 2), (4, 2, 2, 2), 3),  # cuboid_same_many2many shape
     ((16, 16, 16), (3, 3, 3), 3),
     ((12, 10, 8, 2), (3, 3, 3), 3),   # LHS
@@ -231,3 +240,18 @@ def test_large_batch_grid(af):
     one = oracle.convolve(np.ones((4, 4), np.float32), w, 2, False)
     assert got.shape == (4, 4, 65536, 1)
     assert float(got.sum()) == float(one.sum()) * 65536
+
+
+
+def test_strip_long_segments(af, rng):
+    """32 slices of 128 x 4096 give the strip kernel 64-row segments, so its
+    four-block input ring wraps within a workgroup; checked on row slabs at
+    segment edges and at both image borders."""
+    x = rng.random((128, 4096, 32), dtype=np.float32)
+    w = rng.uniform(-1, 1, (7, 7)).astype(np.float32)
+    for mode in (0, 1):
+        got = af.convolve2(x, w, mode, SPATIAL).to_numpy()
+        h = 0 if mode else 3
+        for k in (0, 13, 31):
+            for y0 in (0, 48, 64, 120, 2040, got.shape[1] - 16):
+                slab_check(got, x, k, lambda xs: oracle.convolve(xs, w, 2, True), 7, h, h, y0)
