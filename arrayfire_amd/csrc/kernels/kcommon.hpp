@@ -80,6 +80,20 @@ __device__ inline long long xcd_remap(long long bid, long long nblk) {
     return xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
 }
 
+// 16-byte LDS accesses at addresses known to be 16-byte aligned. The load is
+// volatile so that it stays whole: a load narrowed to the elements used is
+// split into 4-byte reads, which conflict in the LDS banks when lanes are 16
+// bytes apart. p must point into LDS.
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) f32x4_t lds_f32x4_t;
+__device__ __attribute__((always_inline)) inline float4 lds_load16(const float *p) {
+    const f32x4_t t = *(const volatile lds_f32x4_t *)p;
+    return make_float4(t.x, t.y, t.z, t.w);
+}
+__device__ __attribute__((always_inline)) inline void lds_store16(float *p, float4 v) {
+    *(lds_f32x4_t *)p = f32x4_t{v.x, v.y, v.z, v.w};
+}
+
 // Output rows per workgroup of a strip kernel (a multiple of 16, at most
 // 256): long segments amortise the halo rows each segment loads and
 // computes again, short ones keep at least about 2048 workgroups (8 per CU)

@@ -15,6 +15,8 @@
 //    element.
 #include <stdlib.h>
 
+#include <type_traits>
+
 #include "kcommon.hpp"
 #include "../../../include/afhip_kernels.h"
 
@@ -192,9 +194,11 @@ __global__ __launch_bounds__(256) void sep_fused_f32(float *__restrict__ out,
 //    read zeros) and written to xin between the two passes of step u + 1.
 // Sums run in the same order as sep_fused_f32 (flipped taps ascending); taps
 // past the filter length are skipped by uniform branches (EXACT: cf == rf ==
-// F, no branches), never multiplied by 0. The xin origin is the strip's first input column
-// floored to a multiple of 4; the column pass reads its window at the
-// remaining offset.
+// F, no branches), never multiplied by 0. The xin origin is the strip's
+// first input column floored to a multiple of 4; the column pass reads its
+// window with aligned 16-byte loads (lanes 16 bytes apart, so no LDS bank
+// conflicts) and applies the taps at the remaining offset, a compile-time
+// constant in each of the four copies of the step loop.
 struct SepStripParams {
     dim_t S0, S1, O0, O1;
     dim_t nbx, nseg, nblk;
@@ -212,7 +216,10 @@ __global__ __launch_bounds__(256) void sep_strip_f32(float *__restrict__ out,
     constexpr int P = (H + 15) / 16;                    // column-only steps before the first row pass
     constexpr int RING = 16 * (P + 1) <= 32 ? 32 : 64;  // temp rows kept (power of two)
     constexpr int XQ = (64 + H + 3 + 3) / 4;            // float4s loaded per input row
-    constexpr int XP = 4 * XQ;                          // xin row pitch (floats)
+    // xin row pitch (floats): at least the row, and 16 mod 32 so that the two
+    // rows a 16-byte window read pairs (lanes l and l + 32, rows 2 apart) sit
+    // on opposite halves of the 64 banks
+    constexpr int XP = (4 * XQ + 15) / 32 * 32 + 16;
     constexpr int NSLOT = (16 * XQ + 255) / 256;        // 16-byte loads per lane per step
     constexpr int kOOB = 0x7fffffc0;
     __shared__ __attribute__((aligned(16))) float xin[16 * XP];
@@ -269,7 +276,7 @@ __global__ __launch_bounds__(256) void sep_strip_f32(float *__restrict__ out,
     auto commit = [&]() __attribute__((always_inline)) {
 #pragma unroll
         for (int s = 0; s < NSLOT; ++s)
-            if (s_lds[s] >= 0) *reinterpret_cast<float4 *>(xin + s_lds[s]) = pre[s];
+            if (s_lds[s] >= 0) lds_store16(xin + s_lds[s], pre[s]);  // s_lds is a multiple of 4
     };
 
     issue(0);
@@ -279,49 +286,68 @@ __global__ __launch_bounds__(256) void sep_strip_f32(float *__restrict__ out,
 
     const int cr = threadIdx.x >> 4, cx0 = 4 * (threadIdx.x & 15);  // column pass: row, 4 outputs
     const int rx = threadIdx.x & 63, ry0 = 4 * (threadIdx.x >> 6);  // row pass: column, 4 outputs
-    const float *wsrc = xin + cr * XP + cx0 + off;
-    for (int u = 0; u < U; ++u) {
-        {   // column pass: temp row 16u + cr, outputs cx0 .. cx0 + 3
-            float win[4 + H];
+    const float *wsrc = xin + cr * XP + cx0;  // 16-byte aligned; the window starts OFF floats in
+    // One copy of the step loop per column offset, so that the window is read
+    // with aligned 16-byte loads and indexed with compile-time offsets.
+    auto steps = [&](auto offc) __attribute__((always_inline)) {
+        constexpr int OFF = decltype(offc)::value;
+        constexpr int NWQ = (OFF + 4 + H + 3) / 4;  // float4s per column-pass window
+        static_assert(60 + 4 * NWQ <= XP, "window stays in its xin row");
+        for (int u = 0; u < U; ++u) {
+            {   // column pass: temp row 16u + cr, outputs cx0 .. cx0 + 3
+                float win[4 * NWQ];
 #pragma unroll
-            for (int q = 0; q < 4 + H; ++q) win[q] = wsrc[q];
-            float acc[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int k = 0; k < F; ++k)
-                if (EXACT || k < cf) {  // uniform
-                    if constexpr (!EXACT) asm volatile("");  // keeps the branch (no select of products)
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) acc[e] = __builtin_fmaf(win[e + k], wc[k], acc[e]);
+                for (int q = 0; q < NWQ; ++q) {
+                    const float4 f = lds_load16(wsrc + 4 * q);
+                    win[4 * q] = f.x;
+                    win[4 * q + 1] = f.y;
+                    win[4 * q + 2] = f.z;
+                    win[4 * q + 3] = f.w;
                 }
-            const float4 v = make_float4(acc[0], acc[1], acc[2], acc[3]);
-            float *dst = ring + ((16 * u + cr) & (RING - 1)) * 64 + cx0;
-            *reinterpret_cast<float4 *>(dst) = v;
-            *reinterpret_cast<float4 *>(dst + RING * 64) = v;
+                float acc[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int k = 0; k < F; ++k)
+                    if (EXACT || k < cf) {  // uniform
+                        if constexpr (!EXACT) asm volatile("");  // keeps the branch (no select of products)
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) acc[e] = __builtin_fmaf(win[OFF + e + k], wc[k], acc[e]);
+                    }
+                const float4 v = make_float4(acc[0], acc[1], acc[2], acc[3]);
+                float *dst = ring + ((16 * u + cr) & (RING - 1)) * 64 + cx0;
+                *reinterpret_cast<float4 *>(dst) = v;
+                *reinterpret_cast<float4 *>(dst + RING * 64) = v;
+            }
+            __syncthreads();
+            if (u + 1 < U) commit();
+            if (u + 2 < U) issue(u + 2);
+            if (u >= P) {  // row pass: output rows 16(u - P) + ry0 .. + 3 of column rx
+                const int y0 = 16 * (u - P) + ry0;
+                const float *src = ring + (((16 * (u - P)) & (RING - 1)) + ry0) * 64 + rx;
+                float v[4 + H];
+#pragma unroll
+                for (int q = 0; q < 4 + H; ++q) v[q] = src[q * 64];
+                float acc[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int k = 0; k < F; ++k)
+                    if (EXACT || k < rf) {  // uniform
+                        if constexpr (!EXACT) asm volatile("");
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) acc[i] = __builtin_fmaf(v[i + k], wr[k], acc[i]);
+                    }
+                const int gx = ox0 + rx;
+                float *o = out + (dim_t)(oy0 + y0) * O0 + gx;
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (gx < O0 && oy0 + y0 + i < O1) o[(dim_t)i * O0] = acc[i];
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        if (u + 1 < U) commit();
-        if (u + 2 < U) issue(u + 2);
-        if (u >= P) {  // row pass: output rows 16(u - P) + ry0 .. + 3 of column rx
-            const int y0 = 16 * (u - P) + ry0;
-            const float *src = ring + (((16 * (u - P)) & (RING - 1)) + ry0) * 64 + rx;
-            float v[4 + H];
-#pragma unroll
-            for (int q = 0; q < 4 + H; ++q) v[q] = src[q * 64];
-            float acc[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int k = 0; k < F; ++k)
-                if (EXACT || k < rf) {  // uniform
-                    if constexpr (!EXACT) asm volatile("");
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) acc[i] = __builtin_fmaf(v[i + k], wr[k], acc[i]);
-                }
-            const int gx = ox0 + rx;
-            float *o = out + (dim_t)(oy0 + y0) * O0 + gx;
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-                if (gx < O0 && oy0 + y0 + i < O1) o[(dim_t)i * O0] = acc[i];
-        }
-        __syncthreads();
+    };
+    switch (off) {
+        case 0: steps(std::integral_constant<int, 0>{}); break;
+        case 1: steps(std::integral_constant<int, 1>{}); break;
+        case 2: steps(std::integral_constant<int, 2>{}); break;
+        default: steps(std::integral_constant<int, 3>{}); break;
     }
 }
 

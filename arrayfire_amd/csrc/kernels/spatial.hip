@@ -198,11 +198,13 @@ __global__ __launch_bounds__(256) void conv2_tile(float *__restrict__ out,
 //    after s - 1 reads, and the loads of block s + 3 are issued;
 //  * the taps of step s then run from LDS; one barrier ends the step.
 // Rows and 16-byte column groups outside the slice read zeros (buffer loads).
-// Ring column c (input column ix0 + c) sits at row offset 4 + c: a loaded
-// float4 lands whole at 4 + 4q - off, inside the row's margins, so no column
-// test is needed. Each lane computes 4 columns x 2 rows with the flipped
-// filter in scalar registers, summing in the same order as conv2_tile.
-template <int FX, int FY>
+// A ring row holds the input columns from ix0 floored to a multiple of 4, so
+// loaded float4s are stored and read whole and aligned (lanes 16 bytes apart:
+// no LDS bank conflicts); OFF = ix0 mod 4, a compile-time constant, shifts
+// the taps within each lane's window. Each lane computes 4 columns x 2 rows
+// with the flipped filter in scalar registers, summing in the same order as
+// conv2_tile.
+template <int FX, int FY, int OFF>
 __global__ __launch_bounds__(256) void conv2_strip(float *__restrict__ out,
                                                    const float *__restrict__ in,
                                                    const float *__restrict__ filt,
@@ -210,12 +212,15 @@ __global__ __launch_bounds__(256) void conv2_strip(float *__restrict__ out,
     constexpr int BX = 128, TX = 4, TY = 2;
     constexpr int LW = BX + FX - 1;                    // input columns a strip reads
     constexpr int XQ = (LW + 3 + 3) / 4;               // float4s loaded per input row
-    constexpr int LS = 4 + 4 * XQ;                     // ring row pitch (floats)
+    // ring row pitch (floats): at least the row, and 16 mod 32 so that the
+    // two rows a 16-byte read pairs (lanes l and l + 32, rows 2 apart) sit on
+    // opposite halves of the 64 banks
+    constexpr int LS = (4 * XQ + 15) / 32 * 32 + 16;
     constexpr int NSLOT = (16 * XQ + 255) / 256;       // 16-byte loads per lane per block
-    constexpr int NV = (TX + FX - 1 + 3) / 4;          // float4 reads per row segment
+    constexpr int NV = (OFF + TX + FX - 1 + 3) / 4;    // float4 reads per row segment
     constexpr int kOOB = 0x7fffffc0;
     static_assert(FY <= 17, "two ring blocks per output step");
-    static_assert(4 + 4 * 31 + 4 * NV <= LS, "row segment reads stay in the row");
+    static_assert(4 * 31 + 4 * NV <= LS, "row segment reads stay in the row");
     __shared__ __attribute__((aligned(16))) float ring[64 * LS];
 
     const dim_t blk = xcd_remap(blockIdx.x, p.nblk);
@@ -238,7 +243,6 @@ __global__ __launch_bounds__(256) void conv2_strip(float *__restrict__ out,
     const int ox0 = bx * BX, oy0 = sg * p.seg;
     const int ix0 = ox0 + p.sx - (FX - 1), iy0 = oy0 + p.sy - (FY - 1);
     const int a0 = ix0 - (((ix0 % 4) + 4) % 4);        // ix0 floored to a multiple of 4
-    const int off = ix0 - a0;                          // 0..3
     const int nsteps = ((O1 - oy0 < p.seg ? O1 - oy0 : p.seg) + 15) / 16;
     const int nblocks = nsteps + 1;                    // input blocks the segment reads
 
@@ -253,7 +257,7 @@ __global__ __launch_bounds__(256) void conv2_strip(float *__restrict__ out,
         const bool ok = slot < 16 * XQ && gx >= 0 && gx < S0;  // a float4 is wholly in or out
         s_row[s] = ok ? r : -(1 << 28);                      // a dead slot's row is never in range
         s_col[s] = ok ? gx * 4 : 0;
-        s_lds[s] = slot < 16 * XQ ? r * LS + 4 + 4 * q - off : -1;
+        s_lds[s] = slot < 16 * XQ ? r * LS + 4 * q : -1;
     }
     float4 pre[NSLOT], pre1[NSLOT];
     auto issue = [&](float4 (&v)[NSLOT], int bl) __attribute__((always_inline)) {
@@ -268,13 +272,7 @@ __global__ __launch_bounds__(256) void conv2_strip(float *__restrict__ out,
         float *dst = ring + (bl & 3) * (16 * LS);
 #pragma unroll
         for (int s = 0; s < NSLOT; ++s)
-            if (s_lds[s] >= 0) {
-                float *d = dst + s_lds[s];
-                d[0] = v[s].x;
-                d[1] = v[s].y;
-                d[2] = v[s].z;
-                d[3] = v[s].w;
-            }
+            if (s_lds[s] >= 0) lds_store16(dst + s_lds[s], v[s]);  // s_lds is a multiple of 4
     };
 
     issue(pre, 0);
@@ -298,11 +296,11 @@ __global__ __launch_bounds__(256) void conv2_strip(float *__restrict__ out,
         const int rb = 16 * st + ty * TY;              // first input row (segment-relative)
 #pragma unroll
         for (int r = 0; r < TY + FY - 1; ++r) {
-            const float *row = ring + ((rb + r) & 63) * LS + 4 + tx * TX;
+            const float *row = ring + ((rb + r) & 63) * LS + tx * TX;
             float v[NV * 4];
 #pragma unroll
             for (int q = 0; q < NV; ++q) {
-                const float4 f4 = *reinterpret_cast<const float4 *>(row + 4 * q);
+                const float4 f4 = lds_load16(row + 4 * q);
                 v[4 * q + 0] = f4.x;
                 v[4 * q + 1] = f4.y;
                 v[4 * q + 2] = f4.z;
@@ -316,7 +314,7 @@ __global__ __launch_bounds__(256) void conv2_strip(float *__restrict__ out,
                     for (int i = 0; i < TX; ++i)
 #pragma unroll
                         for (int kx = 0; kx < FX; ++kx)
-                            acc[j][i] = __builtin_fmaf(v[i + kx], wf[ky][kx], acc[j][i]);
+                            acc[j][i] = __builtin_fmaf(v[OFF + i + kx], wf[ky][kx], acc[j][i]);
                 }
             }
         }
@@ -434,14 +432,22 @@ static int launch_generic(void *out, const void *in, const void *filt,
     return launch_status();
 }
 
-// Odd square filters up to 9x9 have a strip variant (conv2_strip).
-static Tile2Fn strip2_lookup(dim_t fx, dim_t fy) {
+// Odd square filters up to 9x9 have a strip variant (conv2_strip). Its
+// column offset OFF = (sx - (F - 1)) mod 4 takes one value for DEFAULT
+// (sx = F / 2) and one for EXPAND (sx = 0); only those two are built.
+template <int F>
+static Tile2Fn strip2_of(int off) {
+    constexpr int DEF = ((-(F - 1) / 2) % 4 + 4) % 4, EXP = ((-(F - 1)) % 4 + 4) % 4;
+    return off == DEF ? conv2_strip<F, F, DEF> : off == EXP ? conv2_strip<F, F, EXP> : nullptr;
+}
+static Tile2Fn strip2_lookup(dim_t fx, dim_t fy, int sx) {
     if (fx != fy) return nullptr;
+    const int off = (int)((((sx - (fx - 1)) % 4) + 4) % 4);
     switch (fx) {
-        case 3: return conv2_strip<3, 3>;
-        case 5: return conv2_strip<5, 5>;
-        case 7: return conv2_strip<7, 7>;
-        case 9: return conv2_strip<9, 9>;
+        case 3: return strip2_of<3>(off);
+        case 5: return strip2_of<5>(off);
+        case 7: return strip2_of<7>(off);
+        case 9: return strip2_of<9>(off);
         default: return nullptr;
     }
 }
@@ -472,7 +478,8 @@ static int launch_tile2(float *out, const float *in, const float *filt,
     t.filt_step2 = p.filt_step[2];
     t.filt_step3 = p.filt_step[3];
     t.batch2 = p.batch[2];
-    Tile2Fn strip = strip2_ok(p, in) ? strip2_lookup(p.fd[0], p.fd[1]) : nullptr;
+    const int sx = p.expand ? 0 : (int)(p.fd[0] / 2);
+    Tile2Fn strip = strip2_ok(p, in) ? strip2_lookup(p.fd[0], p.fd[1], sx) : nullptr;
     if (strip) fn = strip;
     t.nbx = (t.O0 + T2_BX - 1) / T2_BX;  // both kernels are 128 columns wide
     t.seg = strip ? strip_rows(t.O1, t.nbx * p.batch[2] * p.batch[3]) : 0;
