@@ -494,6 +494,175 @@ static int launch_tile2(float *out, const float *in, const float *filt,
     return launch_status();
 }
 
+// ---------------------------------------------------------------- FP32 3-D
+struct Tile3Params {
+    dim_t S0, S1, S2, O0, O1, O2;  // signal and output extents of one volume
+    dim_t in_step3, out_step3, filt_step3;
+    dim_t nbx, nby, nbz, nblk;     // blocks along dim0, dim1, dim2, total workgroups
+    int sx, sy, sz;                // centring offsets (0 for EXPAND)
+};
+
+// FP32 rank-3 kernel for small filters: a 32 x 8 x 8 output block per
+// 256-lane workgroup. The (32+FX-1) x (8+FY-1) x (8+FZ-1) input halo is
+// loaded in one batch of 16-byte loads (every slot's load in flight before
+// any LDS store; rows read as aligned float4s, zeros outside the volume) and
+// scattered into LDS so that tile column c is input column ix0 + c. Each lane
+// computes 4 consecutive x outputs on 2 planes, reading each LDS row segment
+// once (ds_read_b128) for all the taps that touch it; the flipped filter
+// lives in SGPRs.
+constexpr int T3_BX = 32, T3_BY = 8, T3_BZ = 8;
+
+template <int FX, int FY, int FZ>
+__global__ __launch_bounds__(256) void conv3_tile(float *__restrict__ out,
+                                                  const float *__restrict__ in,
+                                                  const float *__restrict__ filt,
+                                                  Tile3Params p) {
+    constexpr int LX = T3_BX + FX - 1, LY = T3_BY + FY - 1, LZ = T3_BZ + FZ - 1;
+    // row pitch: 16-byte aligned, and rows 4 apart (lanes l and l + 32 of a
+    // 16-byte read) 32 words apart mod 64, on opposite halves of the banks
+    constexpr int LS = (((LX + 3) & ~3) * 4) % 64 == 32 ? ((LX + 3) & ~3)
+                     : ((((LX + 3) & ~3) + 4) * 4) % 64 == 32 ? ((LX + 3) & ~3) + 4
+                     : ((((LX + 3) & ~3) + 8) * 4) % 64 == 32 ? ((LX + 3) & ~3) + 8 : ((LX + 3) & ~3) + 12;
+    constexpr int NQ = (LX + 6) / 4;              // float4 slots per row (origin floored to 4)
+    constexpr int SLOTS = LZ * LY * NQ;
+    constexpr int NB = (SLOTS + 255) / 256;       // loads per lane, all in flight at once
+    constexpr int NV = (4 + FX - 1 + 3) / 4;      // float4 reads per row segment
+    static_assert(4 * 7 + 4 * NV <= LS, "row segment reads stay in the row");
+    __shared__ __attribute__((aligned(16))) float tile[LZ * LY * LS];
+
+    const dim_t blk = xcd_remap(blockIdx.x, p.nblk);
+    const dim_t bx = blk % p.nbx;
+    dim_t t = blk / p.nbx;
+    const dim_t by = t % p.nby;
+    t /= p.nby;
+    const dim_t bz = t % p.nbz;
+    const dim_t b3 = t / p.nbz;
+    in += b3 * p.in_step3;
+    out += b3 * p.out_step3;
+    filt += b3 * p.filt_step3;
+
+    float wf[FZ][FY][FX];
+#pragma unroll
+    for (int kz = 0; kz < FZ; ++kz)
+#pragma unroll
+        for (int ky = 0; ky < FY; ++ky)
+#pragma unroll
+            for (int kx = 0; kx < FX; ++kx)
+                wf[kz][ky][kx] = filt[((FZ - 1 - kz) * FY + (FY - 1 - ky)) * FX + (FX - 1 - kx)];
+
+    const dim_t ox0 = bx * T3_BX, oy0 = by * T3_BY, oz0 = bz * T3_BZ;
+    const dim_t ix0 = ox0 + p.sx - (FX - 1), iy0 = oy0 + p.sy - (FY - 1), iz0 = oz0 + p.sz - (FZ - 1);
+    const dim_t a0 = ix0 - (((ix0 % 4) + 4) % 4);
+    const int off = (int)(ix0 - a0);
+    const bool vec = (p.S0 & 3) == 0;
+    {
+        float4 v[NB];
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            const int slot = u * 256 + (int)threadIdx.x;
+            const int q = slot % NQ, r = (slot / NQ) % LY, z = slot / (NQ * LY);
+            v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+            const dim_t gx = a0 + 4 * q, gy = iy0 + r, gz = iz0 + z;
+            if (slot < SLOTS && gy >= 0 && gy < p.S1 && gz >= 0 && gz < p.S2) {
+                const float *row = in + (gz * p.S1 + gy) * p.S0;
+                if (vec && gx >= 0 && gx + 3 < p.S0) {
+                    v[u] = *reinterpret_cast<const float4 *>(row + gx);
+                } else {
+                    float e4[4];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) e4[e] = (gx + e >= 0 && gx + e < p.S0) ? row[gx + e] : 0.f;
+                    v[u] = make_float4(e4[0], e4[1], e4[2], e4[3]);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            const int slot = u * 256 + (int)threadIdx.x;
+            const int q = slot % NQ, rz = slot / NQ;
+            const int c0 = 4 * q - off;
+            if (slot < SLOTS) {
+                const float e4[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (c0 + e >= 0 && c0 + e < LS) tile[rz * LS + c0 + e] = c0 + e < LX ? e4[e] : 0.f;
+            }
+        }
+    }
+    __syncthreads();
+
+    const int tx = threadIdx.x & 7, ty = (threadIdx.x >> 3) & 7, tz = threadIdx.x >> 6;  // 8 x 8 x 4
+    float acc[2][4];
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[k][i] = 0.f;
+#pragma unroll
+    for (int zz = 0; zz < 2 + FZ - 1; ++zz)
+#pragma unroll
+        for (int ky = 0; ky < FY; ++ky) {
+            const float *row = tile + ((2 * tz + zz) * LY + ty + ky) * LS + 4 * tx;
+            float w4[4 * NV];
+#pragma unroll
+            for (int q = 0; q < NV; ++q) {
+                const float4 f4 = lds_load16(row + 4 * q);
+                w4[4 * q] = f4.x;
+                w4[4 * q + 1] = f4.y;
+                w4[4 * q + 2] = f4.z;
+                w4[4 * q + 3] = f4.w;
+            }
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int kz = zz - k;
+                if (kz >= 0 && kz < FZ) {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+#pragma unroll
+                        for (int kx = 0; kx < FX; ++kx) acc[k][i] = __builtin_fmaf(w4[i + kx], wf[kz][ky][kx], acc[k][i]);
+                }
+            }
+        }
+
+    const dim_t ox = ox0 + 4 * tx, oy = oy0 + ty;
+    if (oy >= p.O1) return;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const dim_t oz = oz0 + 2 * tz + k;
+        if (oz >= p.O2) continue;
+        float *orow = out + (oz * p.O1 + oy) * p.O0;
+        if ((p.O0 & 3) == 0 && ox + 4 <= p.O0) {
+            *reinterpret_cast<float4 *>(orow + ox) = make_float4(acc[k][0], acc[k][1], acc[k][2], acc[k][3]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (ox + i < p.O0) orow[ox + i] = acc[k][i];
+        }
+    }
+}
+
+static int launch_tile3(float *out, const float *in, const float *filt, const SpatialParams &p, hipStream_t s) {
+    Tile3Params t;
+    t.S0 = p.sd[0];
+    t.S1 = p.sd[1];
+    t.S2 = p.sd[2];
+    t.O0 = p.od[0];
+    t.O1 = p.od[1];
+    t.O2 = p.od[2];
+    t.in_step3 = p.in_step[3];
+    t.out_step3 = p.out_step[3];
+    t.filt_step3 = p.filt_step[3];
+    t.nbx = (t.O0 + T3_BX - 1) / T3_BX;
+    t.nby = (t.O1 + T3_BY - 1) / T3_BY;
+    t.nbz = (t.O2 + T3_BZ - 1) / T3_BZ;
+    t.nblk = t.nbx * t.nby * t.nbz * p.batch[3];
+    t.sx = p.expand ? 0 : (int)(p.fd[0] / 2);
+    t.sy = p.expand ? 0 : (int)(p.fd[1] / 2);
+    t.sz = p.expand ? 0 : (int)(p.fd[2] / 2);
+    if (t.nblk == 0) return K_OK;
+    if (t.nblk > 0x7fffffffLL) return K_ERR_SIZE;
+    conv3_tile<3, 3, 3><<<dim3((unsigned)t.nblk), dim3(256), 0, s>>>(out, in, filt, t);
+    return launch_status();
+}
+
 }  // namespace afhip
 
 using namespace afhip;
@@ -510,6 +679,9 @@ extern "C" int afhip_k_convolve(int dtype, void *out, const long long *odims,
         if (Tile2Fn fn = tile2_lookup(fdims[0], fdims[1]))
             return launch_tile2((float *)out, (const float *)in, (const float *)filt, p, fn, s);
     }
+    if (dtype == DT_F32 && rank == 3 && p.batch[1] == 1 && p.batch[2] == 1 && fdims[0] == 3 && fdims[1] == 3 &&
+        fdims[2] == 3)
+        return launch_tile3((float *)out, (const float *)in, (const float *)filt, p, s);
     switch (dtype) {
         case DT_F32: return launch_generic<float, float>(out, in, filt, p, s);
         case DT_F64: return launch_generic<double, double>(out, in, filt, p, s);

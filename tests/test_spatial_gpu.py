@@ -44,6 +44,12 @@ CASES = [
     ((16, 16, 16), (3, 3, 3), 3),
     ((12, 10, 8, 2), (3, 3, 3), 3),   # LHS
     ((9, 8, 7), (3, 2, 4, 3), 3),     # RHS
+    # 3x3x3 takes the FP32 3-D tile kernel: partial blocks in every dimension,
+    # rows that are and are not whole 16-byte vectors, every batch kind
+    ((70, 20, 19), (3, 3, 3), 3),
+    ((45, 33, 20), (3, 3, 3), 3),
+    ((9, 8, 7), (3, 3, 3, 2), 3),     # RHS
+    ((10, 11, 6, 2), (3, 3, 3, 2), 3),  # SAME
 ]
 
 
