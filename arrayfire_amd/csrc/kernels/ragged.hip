@@ -52,28 +52,53 @@ struct RGeo {
     static constexpr int NTY = RG_THREADS / NTX;
     static constexpr int BX = TX * NTX, BY = TY * NTY;
     static constexpr int LW = BX + RG_MAXF - 1, LH = BY + RG_MAXF - 1;
-    static constexpr int LS = (LW + 3) & ~3;
+    // row pitch: 16-byte aligned and 4 mod 8 floats, so that the rows a
+    // 16-byte read pairs (lanes l and l + 32, 8 rows apart) sit 32 words
+    // apart mod 64, on opposite halves of the LDS banks
+    static constexpr int LS0 = (LW + 3) & ~3;
+    static constexpr int LS = LS0 % 8 == 0 ? LS0 + 4 : LS0;
 };
 
 // acc[j][i] for output (tx*TX + i, ty*TY + j) of the tile; tile(lx+kx, ly+ky)
-// * wf(ky, kx) with wf the flipped filter (row pitch FX) in LDS.
+// * wf(ky, kx) with wf the flipped filter (row pitch RG_WP) in LDS. Row
+// segments (TX a multiple of 4) and filter rows are read as whole 16-byte
+// vectors: one broadcast read per 4 taps instead of one per tap.
+constexpr int RG_WP = 16;  // filter row pitch in LDS (floats), >= RG_MAXF
 template <int FX, int V>
 __device__ inline void tile_fx(const float *tile, const float *wf, int FY, int LS,
                                float (&acc)[RGeo<V>::TY][RGeo<V>::TX]) {
     constexpr int TX = RGeo<V>::TX, TY = RGeo<V>::TY, NTX = RGeo<V>::NTX;
+    constexpr int NV = (TX + FX - 1 + 3) / 4, NW = (FX + 3) / 4;
     const int tx = threadIdx.x % NTX, ty = threadIdx.x / NTX;
     const float *base = tile + (ty * TY) * LS + tx * TX;
     for (int r = 0; r < TY + FY - 1; ++r) {
-        float v[TX + FX - 1];
+        float v[4 * NV];
+        if constexpr (TX % 4 == 0) {
 #pragma unroll
-        for (int c = 0; c < TX + FX - 1; ++c) v[c] = base[r * LS + c];
+            for (int q = 0; q < NV; ++q) {
+                const float4 f4 = lds_load16(base + r * LS + 4 * q);
+                v[4 * q] = f4.x;
+                v[4 * q + 1] = f4.y;
+                v[4 * q + 2] = f4.z;
+                v[4 * q + 3] = f4.w;
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < TX + FX - 1; ++c) v[c] = base[r * LS + c];
+        }
 #pragma unroll
         for (int j = 0; j < TY; ++j) {
             const int ky = r - j;
             if (ky >= 0 && ky < FY) {
-                float w[FX];
+                float w[4 * NW];
 #pragma unroll
-                for (int kx = 0; kx < FX; ++kx) w[kx] = wf[ky * FX + kx];
+                for (int q = 0; q < NW; ++q) {
+                    const float4 f4 = lds_load16(wf + ky * RG_WP + 4 * q);
+                    w[4 * q] = f4.x;
+                    w[4 * q + 1] = f4.y;
+                    w[4 * q + 2] = f4.z;
+                    w[4 * q + 3] = f4.w;
+                }
 #pragma unroll
                 for (int i = 0; i < TX; ++i)
 #pragma unroll
@@ -94,7 +119,7 @@ __device__ inline void tile_any(const float *tile, const float *wf, int FX, int 
     for (int j = 0; j < TY; ++j)
         for (int ky = 0; ky < FY; ++ky)
             for (int kx = 0; kx < FX; ++kx) {
-                const float w = wf[ky * FX + kx];
+                const float w = wf[ky * RG_WP + kx];
 #pragma unroll
                 for (int i = 0; i < TX; ++i)
                     acc[j][i] = __builtin_fmaf(base[(j + ky) * LS + i + kx], w, acc[j][i]);
@@ -107,7 +132,7 @@ __global__ __launch_bounds__(RG_THREADS) void ragged_conv2(const RaggedProblem *
                                                            long long nitems) {
     using G = RGeo<V>;
     __shared__ __attribute__((aligned(16))) float tile[G::LH * G::LS];
-    __shared__ float wf[RG_MAXF * RG_MAXF];
+    __shared__ __attribute__((aligned(16))) float wf[RG_MAXF * RG_WP];
     for (long long it = blockIdx.x; it < nitems; it += gridDim.x) {
         const int2 item = items[it];
         const RaggedProblem pr = probs[item.x];
@@ -115,9 +140,9 @@ __global__ __launch_bounds__(RG_THREADS) void ragged_conv2(const RaggedProblem *
         const long long ox0 = (long long)bx * G::BX, oy0 = (long long)by * G::BY;
         const int LW = G::BX + pr.FX - 1, LH = G::BY + pr.FY - 1;
         __syncthreads();  // previous item's tile and filter no longer read
-        for (int e = threadIdx.x; e < pr.FX * pr.FY; e += RG_THREADS) {
-            const int ky = e / pr.FX, kx = e - (e / pr.FX) * pr.FX;
-            wf[e] = pr.filt[(pr.FY - 1 - ky) * pr.FX + (pr.FX - 1 - kx)];
+        for (int e = threadIdx.x; e < pr.FY * RG_WP; e += RG_THREADS) {
+            const int ky = e / RG_WP, kx = e % RG_WP;  // padding taps (kx >= FX) are never read
+            wf[e] = kx < pr.FX ? pr.filt[(pr.FY - 1 - ky) * pr.FX + (pr.FX - 1 - kx)] : 0.f;
         }
         load_halo_tile_f32<RG_THREADS, (G::LW + 6) / 4>(tile, pr.in, pr.S0, pr.S1,
                                                         ox0 + pr.sx - (pr.FX - 1),
