@@ -31,7 +31,10 @@ XD = (56, 56, 64, 32)
 WD = (3, 3, 64, 128)
 ST, PD, DL = (1, 1), (1, 1), (1, 1)
 SEED = 20261202 + 2
-PEAK = {"hbm_GBs": 8000.0, "bf16_TFs": 2500.0, "fp32_TFs": 157.3}
+PEAK = {"hbm_GBs": 8000.0, "bf16_TFs": 2500.0, "fp32_TFs": 157.3,
+        # FP32 FMAs on the vector ALUs without packed FP32 instructions (not
+        # used here): half of the 157.3 TFLOP/s
+        "fp32_valu_TFs": 78.6}
 # Mangled names of the step's kernels (as rocprofv3 reports them).
 KERNEL_SYMBOL = {
     "conv_fwd": "afhip::conv_pipe<1, unsigned short, unsigned short, 2, 3, 0, 3>",
@@ -329,8 +332,11 @@ def spatial_secondary(af, ev, stream, reps=10):
 
     def entry(name, ms, nbytes, nflops, npix, extra=None):
         gbs = nbytes / (ms * 1e-3) / 1e9
+        tfs = nflops / (ms * 1e-3) / 1e12
         d = {"ms": round(ms, 4), "GB_s": round(gbs, 1), "hbm_frac": round(gbs / PEAK["hbm_GBs"], 3),
-             "TFLOP_s": round(nflops / (ms * 1e-3) / 1e12, 2),
+             "TFLOP_s": round(tfs, 2), "valu_frac": round(tfs / PEAK["fp32_valu_TFs"], 3),
+             # the binding roofline: the larger of the HBM and VALU times at peak
+             "roofline_frac": round(max(gbs / PEAK["hbm_GBs"], tfs / PEAK["fp32_valu_TFs"]), 3),
              "GPix_s": round(npix / (ms * 1e-3) / 1e9, 2)}
         if extra:
             d.update(extra)
@@ -408,6 +414,7 @@ def ragged_secondary(af, ev, stream):
             "TFLOP_s": round(flops / (ms_graph * 1e-3) / 1e12, 2),
             "GB_s": round(bytes_ / (ms_graph * 1e-3) / 1e9, 1),
             "hbm_frac": round(bytes_ / (ms_graph * 1e-3) / 1e9 / PEAK["hbm_GBs"], 3),
+            "valu_frac": round(flops / (ms_graph * 1e-3) / 1e12 / PEAK["fp32_valu_TFs"], 3),
             "algorithmic_flops": flops, "algorithmic_bytes": bytes_,
             "plan_setup_s_incl_autotune": round(setup_s, 2)}
 
