@@ -253,7 +253,7 @@ __global__ __launch_bounds__(256) void sep_strip_f32(float *__restrict__ out,
     // this lane's load slots: row s_row, byte offset s_col in the image row
     const auto rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(img), (short)0, S0 * S1 * 4,
                                                         0x00020000);
-    int s_row[NSLOT], s_col[NSLOT], s_lds[NSLOT];
+    int s_row[NSLOT], s_col[NSLOT], s_lds[NSLOT], s_off[NSLOT];
 #pragma unroll
     for (int s = 0; s < NSLOT; ++s) {
         const int slot = s * 256 + (int)threadIdx.x;
@@ -262,15 +262,26 @@ __global__ __launch_bounds__(256) void sep_strip_f32(float *__restrict__ out,
         const bool ok = slot < 16 * XQ && gx >= 0 && gx < S0;  // a float4 is wholly in or out
         s_row[s] = ok ? r : -(1 << 28);                      // a dead slot's row is never in range
         s_col[s] = ok ? gx * 4 : 0;
+        s_off[s] = ok ? r * (S0 * 4) + gx * 4 : kOOB;        // from the block's first row
         s_lds[s] = slot < 16 * XQ ? r * XP + 4 * q : -1;
     }
     float4 pre[NSLOT];
     auto issue = [&](int u) __attribute__((always_inline)) {
+        const int row0 = iy0 + 16 * u;
+        if (row0 >= 0 && row0 + 16 <= S1) {
+            // the block's 16 rows are inside the image: the row goes in the
+            // uniform offset, the lane offsets are fixed
 #pragma unroll
-        for (int s = 0; s < NSLOT; ++s) {
-            const int gy = iy0 + 16 * u + s_row[s];
-            const int o = (unsigned)gy < (unsigned)S1 ? gy * (S0 * 4) + s_col[s] : kOOB;
-            pre[s] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rsrc, o, 0, 0));
+            for (int s = 0; s < NSLOT; ++s)
+                pre[s] = __builtin_bit_cast(
+                    float4, __builtin_amdgcn_raw_buffer_load_b128(rsrc, s_off[s], row0 * (S0 * 4), 0));
+        } else {
+#pragma unroll
+            for (int s = 0; s < NSLOT; ++s) {
+                const int gy = row0 + s_row[s];
+                const int o = (unsigned)gy < (unsigned)S1 ? gy * (S0 * 4) + s_col[s] : kOOB;
+                pre[s] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rsrc, o, 0, 0));
+            }
         }
     };
     auto commit = [&]() __attribute__((always_inline)) {
@@ -285,7 +296,12 @@ __global__ __launch_bounds__(256) void sep_strip_f32(float *__restrict__ out,
     __syncthreads();
 
     const int cr = threadIdx.x >> 4, cx0 = 4 * (threadIdx.x & 15);  // column pass: row, 4 outputs
-    const int rx = threadIdx.x & 63, ry0 = 4 * (threadIdx.x >> 6);  // row pass: column, 4 outputs
+    const int rx = threadIdx.x & 63;                                  // row pass: column, 4 outputs
+    const int ry0 = 4 * __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);  // wave-uniform
+    // results: buffer stores at a fixed lane offset (past the end for columns
+    // beyond the output) and a uniform row offset
+    const auto orsrc = __builtin_amdgcn_make_buffer_rsrc(out, (short)0, O0 * O1 * 4, 0x00020000);
+    const int o_lane = ox0 + rx < O0 ? (ox0 + rx) * 4 : kOOB;
     const float *wsrc = xin + cr * XP + cx0;  // 16-byte aligned; the window starts OFF floats in
     // One copy of the step loop per column offset, so that the window is read
     // with aligned 16-byte loads and indexed with compile-time offsets.
@@ -334,11 +350,11 @@ __global__ __launch_bounds__(256) void sep_strip_f32(float *__restrict__ out,
 #pragma unroll
                         for (int i = 0; i < 4; ++i) acc[i] = __builtin_fmaf(v[i + k], wr[k], acc[i]);
                     }
-                const int gx = ox0 + rx;
-                float *o = out + (dim_t)(oy0 + y0) * O0 + gx;
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
-                    if (gx < O0 && oy0 + y0 + i < O1) o[(dim_t)i * O0] = acc[i];
+                    if (oy0 + y0 + i < O1)  // uniform
+                        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, acc[i]), orsrc, o_lane,
+                                                              (oy0 + y0 + i) * (O0 * 4), 0);
             }
             __syncthreads();
         }
@@ -352,15 +368,16 @@ __global__ __launch_bounds__(256) void sep_strip_f32(float *__restrict__ out,
 }
 
 // The strip kernel takes images whose rows are whole 16-byte vectors from a
-// 16-byte aligned base, below 2 GiB each (its buffer offsets are 32-bit).
+// 16-byte aligned base, input and output images below 2 GiB each (its buffer
+// offsets are 32-bit).
 // AFHIP_SEP_STRIP=0 in the environment selects the tile kernel instead.
-static bool sep_strip_ok(const long long *sdims, const void *in) {
+static bool sep_strip_ok(const long long *sdims, const long long *odims, const void *in) {
     static const bool enabled = [] {
         const char *e = getenv("AFHIP_SEP_STRIP");
         return !(e && e[0] == '0');
     }();
     return enabled && sdims[0] % 4 == 0 && ((uintptr_t)in & 15) == 0 &&
-           sdims[0] * sdims[1] * 4 < 0x7fffffc0LL;
+           sdims[0] * sdims[1] * 4 < 0x7fffffc0LL && odims[0] * odims[1] * 4 < 0x7fffffc0LL;
 }
 
 static unsigned grid_for(dim_t n) {
@@ -408,7 +425,7 @@ extern "C" int afhip_k_convolve2_sep(int dtype, void *out, const long long *odim
     p.expand = expand;
     p.flen = 0;
     if (dtype == DT_F32 && tmp == nullptr && cflen <= SF_MAXF && rflen <= SF_MAXF &&
-        sep_strip_ok(sdims, in)) {
+        sep_strip_ok(sdims, odims, in)) {
         SepStripParams f;
         f.S0 = sdims[0];
         f.S1 = sdims[1];
