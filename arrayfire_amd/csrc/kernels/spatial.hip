@@ -248,7 +248,7 @@ __global__ __launch_bounds__(256) void conv2_strip(float *__restrict__ out,
 
     const auto rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(img), (short)0, S0 * S1 * 4,
                                                         0x00020000);
-    int s_row[NSLOT], s_col[NSLOT], s_lds[NSLOT];
+    int s_row[NSLOT], s_col[NSLOT], s_lds[NSLOT], s_off[NSLOT];
 #pragma unroll
     for (int s = 0; s < NSLOT; ++s) {
         const int slot = s * 256 + (int)threadIdx.x;
@@ -257,15 +257,26 @@ __global__ __launch_bounds__(256) void conv2_strip(float *__restrict__ out,
         const bool ok = slot < 16 * XQ && gx >= 0 && gx < S0;  // a float4 is wholly in or out
         s_row[s] = ok ? r : -(1 << 28);                      // a dead slot's row is never in range
         s_col[s] = ok ? gx * 4 : 0;
+        s_off[s] = ok ? r * (S0 * 4) + gx * 4 : kOOB;        // from the block's first row
         s_lds[s] = slot < 16 * XQ ? r * LS + 4 * q : -1;
     }
     float4 pre[NSLOT], pre1[NSLOT];
     auto issue = [&](float4 (&v)[NSLOT], int bl) __attribute__((always_inline)) {
+        const int row0 = iy0 + 16 * bl;
+        if (row0 >= 0 && row0 + 16 <= S1) {
+            // the block's 16 rows are inside the image: the row goes in the
+            // uniform offset, the lane offsets are fixed
 #pragma unroll
-        for (int s = 0; s < NSLOT; ++s) {
-            const int gy = iy0 + 16 * bl + s_row[s];
-            const int o = (unsigned)gy < (unsigned)S1 ? gy * (S0 * 4) + s_col[s] : kOOB;
-            v[s] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rsrc, o, 0, 0));
+            for (int s = 0; s < NSLOT; ++s)
+                v[s] = __builtin_bit_cast(
+                    float4, __builtin_amdgcn_raw_buffer_load_b128(rsrc, s_off[s], row0 * (S0 * 4), 0));
+        } else {
+#pragma unroll
+            for (int s = 0; s < NSLOT; ++s) {
+                const int gy = row0 + s_row[s];
+                const int o = (unsigned)gy < (unsigned)S1 ? gy * (S0 * 4) + s_col[s] : kOOB;
+                v[s] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rsrc, o, 0, 0));
+            }
         }
     };
     auto commit = [&](const float4 (&v)[NSLOT], int bl) __attribute__((always_inline)) {
