@@ -443,22 +443,31 @@ static int launch_generic(void *out, const void *in, const void *filt,
     return launch_status();
 }
 
-// Odd square filters up to 9x9 have a strip variant (conv2_strip). Its
-// column offset OFF = (sx - (F - 1)) mod 4 takes one value for DEFAULT
-// (sx = F / 2) and one for EXPAND (sx = 0); only those two are built.
-template <int F>
+// Odd filter sides 3..9 have a strip variant (conv2_strip). Its column
+// offset OFF = (sx - (FX - 1)) mod 4 takes one value for DEFAULT (sx = FX / 2)
+// and one for EXPAND (sx = 0); only those two are built.
+template <int FX, int FY>
 static Tile2Fn strip2_of(int off) {
-    constexpr int DEF = ((-(F - 1) / 2) % 4 + 4) % 4, EXP = ((-(F - 1)) % 4 + 4) % 4;
-    return off == DEF ? conv2_strip<F, F, DEF> : off == EXP ? conv2_strip<F, F, EXP> : nullptr;
+    constexpr int DEF = ((-(FX - 1) / 2) % 4 + 4) % 4, EXP = ((-(FX - 1)) % 4 + 4) % 4;
+    return off == DEF ? conv2_strip<FX, FY, DEF> : off == EXP ? conv2_strip<FX, FY, EXP> : nullptr;
+}
+template <int FX>
+static Tile2Fn strip2_fy(dim_t fy, int off) {
+    switch (fy) {
+        case 3: return strip2_of<FX, 3>(off);
+        case 5: return strip2_of<FX, 5>(off);
+        case 7: return strip2_of<FX, 7>(off);
+        case 9: return strip2_of<FX, 9>(off);
+        default: return nullptr;
+    }
 }
 static Tile2Fn strip2_lookup(dim_t fx, dim_t fy, int sx) {
-    if (fx != fy) return nullptr;
     const int off = (int)((((sx - (fx - 1)) % 4) + 4) % 4);
     switch (fx) {
-        case 3: return strip2_of<3>(off);
-        case 5: return strip2_of<5>(off);
-        case 7: return strip2_of<7>(off);
-        case 9: return strip2_of<9>(off);
+        case 3: return strip2_fy<3>(fy, off);
+        case 5: return strip2_fy<5>(fy, off);
+        case 7: return strip2_fy<7>(fy, off);
+        case 9: return strip2_fy<9>(fy, off);
         default: return nullptr;
     }
 }
@@ -475,8 +484,10 @@ static bool strip2_ok(const SpatialParams &p, const void *in) {
            ((uintptr_t)in & 15) == 0 && p.sd[0] * p.sd[1] * 4 < 0x7fffffc0LL;
 }
 
+// fn: the conv2_tile variant (may be null when strip is not); strip: the
+// conv2_strip variant, used when not null.
 static int launch_tile2(float *out, const float *in, const float *filt,
-                        const SpatialParams &p, Tile2Fn fn, hipStream_t s) {
+                        const SpatialParams &p, Tile2Fn fn, Tile2Fn strip, hipStream_t s) {
     Tile2Params t;
     t.S0 = p.sd[0];
     t.S1 = p.sd[1];
@@ -489,8 +500,6 @@ static int launch_tile2(float *out, const float *in, const float *filt,
     t.filt_step2 = p.filt_step[2];
     t.filt_step3 = p.filt_step[3];
     t.batch2 = p.batch[2];
-    const int sx = p.expand ? 0 : (int)(p.fd[0] / 2);
-    Tile2Fn strip = strip2_ok(p, in) ? strip2_lookup(p.fd[0], p.fd[1], sx) : nullptr;
     if (strip) fn = strip;
     t.nbx = (t.O0 + T2_BX - 1) / T2_BX;  // both kernels are 128 columns wide
     t.seg = strip ? strip_rows(t.O1, t.nbx * p.batch[2] * p.batch[3]) : 0;
@@ -687,8 +696,10 @@ extern "C" int afhip_k_convolve(int dtype, void *out, const long long *odims,
     fill_params(p, odims, sdims, fdims, kind, rank, expand);
     hipStream_t s = (hipStream_t)stream;
     if (dtype == DT_F32 && rank == 2 && p.batch[1] == 1) {
-        if (Tile2Fn fn = tile2_lookup(fdims[0], fdims[1]))
-            return launch_tile2((float *)out, (const float *)in, (const float *)filt, p, fn, s);
+        const int sx = expand ? 0 : (int)(fdims[0] / 2);
+        Tile2Fn strip = strip2_ok(p, in) ? strip2_lookup(fdims[0], fdims[1], sx) : nullptr;
+        Tile2Fn fn = tile2_lookup(fdims[0], fdims[1]);
+        if (fn || strip) return launch_tile2((float *)out, (const float *)in, (const float *)filt, p, fn, strip, s);
     }
     if (dtype == DT_F32 && rank == 3 && p.batch[1] == 1 && p.batch[2] == 1 && fdims[0] == 3 && fdims[1] == 3 &&
         fdims[2] == 3)

@@ -40,6 +40,10 @@ CASES = [
     ((36, 20, 2, 2), (3, 3, 2, 2), 2),  # SAME
     ((1024, 24), (7, 7), 2),
     ((64, 600), (5, 5), 2),
+    ((128, 300), (7, 3), 2),          # odd non-square filters: strip kernel too
+    ((64, 70, 2), (3, 9), 2),         # LHS
+    ((100, 40), (9, 5, 2), 2),        # RHS
+    ((68, 300), (5, 7), 2),
     ((10, 11, 2, 2), (4, 2, 2, 2), 3),  # cuboid_same_many2many shape
     ((16, 16, 16), (3, 3, 3), 3),
     ((12, 10, 8, 2), (3, 3, 3), 3),   # LHS
