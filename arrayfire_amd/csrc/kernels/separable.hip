@@ -13,6 +13,9 @@
 //    second LDS tile of 64 x (64+F-1) (rounded to float, the reference's temp
 //    precision), then the row pass to HBM: one read and one write of every
 //    element.
+//  * sep_strip_f32<F>: the same two passes walked down 64-column strips with
+//    a temp-row ring in LDS, for rows of whole 16-byte vectors (see the
+//    kernel).
 #include <stdlib.h>
 
 #include <type_traits>

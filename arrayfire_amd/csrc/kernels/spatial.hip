@@ -5,7 +5,11 @@
 // zero outside x, accumulated in AccT, stored as InT(acc); batch stepping as
 // convolve_nd (:132-211). The host passes per-kind batch extents and steps.
 //
-// Two implementations:
+// Implementations:
+//  * conv2_strip<FX,FY,OFF>: FP32 rank-2 kernel for odd filter sides 3..9 on
+//    rows of whole 16-byte vectors; walks 128-column strips down the image
+//    with an input-row ring in LDS and register prefetch (see the kernel).
+//  * conv3_tile<3,3,3>: FP32 rank-3 kernel, 32x8x8 output blocks.
 //  * conv2_tile<FX,FY>: FP32 rank-2 kernel. A 128x64 output tile per
 //    256-thread workgroup; the (128+FX-1)x(64+FY-1) input halo tile is staged
 //    in LDS with zero borders so the tap loops are branch-free; each lane keeps
