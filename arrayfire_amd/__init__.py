@@ -71,7 +71,7 @@ EXPORTED = [
     "afhip_convolve2_gradient_nn_ex",
     # include/afhip_kernels.h
     "afhip_k_convolve", "afhip_k_convolve2_sep", "afhip_k_cast", "afhip_k_fill",
-    "afhip_k_scale", "afhip_k_conv2_nn_fwd", "afhip_k_conv2_nn_dgrad",
+    "afhip_k_scale", "afhip_k_scale_complex", "afhip_k_conv2_nn_fwd", "afhip_k_conv2_nn_dgrad",
     "afhip_k_conv2_nn_wgrad_workspace", "afhip_k_conv2_nn_wgrad",
     "afhip_k_conv2_nn_f64", "afhip_k_conv2_nn_patch_workspace", "afhip_k_conv2_nn_patch",
     "afhip_k_conv2_nn_wgrad_rows_workspace", "afhip_k_conv2_nn_wgrad_rows",

@@ -88,9 +88,16 @@ int autotune_pick(const std::string &key, int ncand, const std::function<void(in
     const int known = autotune_lookup(key);
     if (known >= 0 && known < ncand) return known;
     hipStream_t s = stream();
-    hipEvent_t e0, e1;
-    check_hip(hipEventCreate(&e0), "hipEventCreate");
-    check_hip(hipEventCreate(&e1), "hipEventCreate");
+    // Destroyed on every exit, including a launch that throws.
+    struct Event {
+        hipEvent_t e = nullptr;
+        Event() { check_hip(hipEventCreate(&e), "hipEventCreate"); }
+        ~Event() {
+            if (e) (void)hipEventDestroy(e);
+        }
+    };
+    Event ev0, ev1;
+    hipEvent_t e0 = ev0.e, e1 = ev1.e;
     int best = 0;
     float best_ms = 1e30f;
     for (int c = 0; c < ncand; ++c) {
@@ -106,8 +113,6 @@ int autotune_pick(const std::string &key, int ncand, const std::function<void(in
             best = c;
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     std::lock_guard<std::mutex> g(g_m);
     load_locked();
     (*g_cache)[key] = best;

@@ -119,13 +119,13 @@ static af_array convolve(const ArrayImpl &sig, const ArrayImpl &filt, BatchKind 
     const af_dtype at = acc_type(sig.type);
     auto fbuf = cast_buffer(filt, at);
     const dim4 od = conv_out_dims(sig.dims, filt.dims, kind, rank, expand);
-    af_array out = make_handle(sig.type, od);
+    NewHandle out(make_handle(sig.type, od));
     if (od.elements() > 0) {
-        check_k(afhip_k_convolve(sig.type, get(out).data(), od.d, sig.data(), sig.dims.d, fbuf->ptr,
+        check_k(afhip_k_convolve(sig.type, out.data(), od.d, sig.data(), sig.dims.d, fbuf->ptr,
                                  filt.dims.d, (int)kind, rank, expand ? 1 : 0, stream()),
                 "convolve");
     }
-    return out;
+    return out.release();
 }
 
 // cpu::convolve2<T, accT> (separable), src/backend/cpu/convolve.cpp:68-99
@@ -138,8 +138,8 @@ static af_array convolve2_sep(const ArrayImpl &sig, const std::shared_ptr<Buffer
         od[1] += rflen - 1;
         td[0] += cflen - 1;
     }
-    af_array out = make_handle(sig.type, od);
-    if (od.elements() == 0) return out;
+    NewHandle out(make_handle(sig.type, od));
+    if (od.elements() == 0) return out.release();
     // FP32 with filters of up to AFHIP_SEP_FUSED_MAX_TAPS runs fused (the
     // intermediate never leaves the chip); everything else runs the two
     // passes through the reference's T-typed temp.
@@ -147,11 +147,11 @@ static af_array convolve2_sep(const ArrayImpl &sig, const std::shared_ptr<Buffer
     const bool fused = sig.type == f32 && cflen <= AFHIP_SEP_FUSED_MAX_TAPS &&
                        rflen <= AFHIP_SEP_FUSED_MAX_TAPS;
     if (!fused) tmp = scratch((size_t)td.elements() * dtype_size(sig.type));
-    check_k(afhip_k_convolve2_sep(sig.type, get(out).data(), od.d, sig.data(), sig.dims.d, cf->ptr,
+    check_k(afhip_k_convolve2_sep(sig.type, out.data(), od.d, sig.data(), sig.dims.d, cf->ptr,
                                   cflen, rf->ptr, rflen, expand ? 1 : 0, tmp ? tmp->ptr : nullptr,
                                   stream()),
             "convolve2_sep");
-    return out;
+    return out.release();
 }
 
 // convolve2_unwrap output size, src/backend/cpu/convolve.cpp:130-135
@@ -221,59 +221,59 @@ static af_dtype nn_out_type(af_dtype in, af_dtype requested) {
 // cpu::convolve2<T> (NN forward), src/backend/cpu/convolve.cpp:160-167
 static af_array conv2_nn(const ArrayImpl &x, const ArrayImpl &w, const NNGeom &g, int precision,
                          af_dtype out_type) {
-    af_array out = make_handle(out_type, g.y);
-    if (g.y.elements() == 0) return out;
+    NewHandle out(make_handle(out_type, g.y));
+    if (g.y.elements() == 0) return out.release();
     if (x.type == f64) {
-        check_k(afhip_k_conv2_nn_f64(0, get(out).data(), x.data(), w.data(), g.x.d, g.w.d, g.y.d, g.st,
+        check_k(afhip_k_conv2_nn_f64(0, out.data(), x.data(), w.data(), g.x.d, g.w.d, g.y.d, g.st,
                                      g.pd, g.dl, stream()),
                 "convolve2_nn");
-        return out;
+        return out.release();
     }
-    if (run_patch(0, x, w, out_type, precision, get(out).data(), g)) return out;
-    check_k(afhip_k_conv2_nn_fwd(x.type, out_type, precision, get(out).data(), x.data(), w.data(), g.x.d,
+    if (run_patch(0, x, w, out_type, precision, out.data(), g)) return out.release();
+    check_k(afhip_k_conv2_nn_fwd(x.type, out_type, precision, out.data(), x.data(), w.data(), g.x.d,
                                  g.w.d, g.y.d, g.st, g.pd, g.dl, stream()),
             "convolve2_nn");
-    return out;
+    return out.release();
 }
 
 // cpu::conv2DataGradient<T>, src/backend/cpu/convolve.cpp:179-211
 static af_array conv2_data_grad(const ArrayImpl &dy, const ArrayImpl &x, const ArrayImpl &w,
                                 const NNGeom &g, int precision, af_dtype out_type) {
-    af_array out = make_handle(out_type, g.x);
-    if (g.x.elements() == 0) return out;
-    void *o = get(out).data();
+    NewHandle out(make_handle(out_type, g.x));
+    if (g.x.elements() == 0) return out.release();
+    void *o = out.data();
     if (dy.elements() == 0 || g.w.elements() == 0) {
         check_k(afhip_k_fill(out_type, o, 0.0, g.x.elements(), stream()), "fill");
-        return out;
+        return out.release();
     }
     if (x.type == f64) {
         check_k(afhip_k_conv2_nn_f64(1, o, dy.data(), w.data(), g.x.d, g.w.d, g.y.d, g.st, g.pd, g.dl,
                                      stream()),
                 "convolve2_gradient_nn");
-        return out;
+        return out.release();
     }
-    if (run_patch(1, dy, w, out_type, precision, o, g)) return out;
+    if (run_patch(1, dy, w, out_type, precision, o, g)) return out.release();
     check_k(afhip_k_conv2_nn_dgrad(x.type, out_type, precision, o, dy.data(), w.data(), g.x.d, g.w.d,
                                    g.y.d, g.st, g.pd, g.dl, stream()),
             "convolve2_gradient_nn");
-    return out;
+    return out.release();
 }
 
 // cpu::conv2FilterGradient<T>, src/backend/cpu/convolve.cpp:213-243
 static af_array conv2_filter_grad(const ArrayImpl &dy, const ArrayImpl &x, const ArrayImpl &w,
                                   const NNGeom &g, int precision, af_dtype out_type) {
-    af_array out = make_handle(out_type, g.w);
-    if (g.w.elements() == 0) return out;
-    void *o = get(out).data();
+    NewHandle out(make_handle(out_type, g.w));
+    if (g.w.elements() == 0) return out.release();
+    void *o = out.data();
     if (dy.elements() == 0 || g.x.elements() == 0) {
         check_k(afhip_k_fill(out_type, o, 0.0, g.w.elements(), stream()), "fill");
-        return out;
+        return out.release();
     }
     if (x.type == f64) {
         check_k(afhip_k_conv2_nn_f64(2, o, x.data(), dy.data(), g.x.d, g.w.d, g.y.d, g.st, g.pd, g.dl,
                                      stream()),
                 "convolve2_gradient_nn");
-        return out;
+        return out.release();
     }
     const char *env = std::getenv("AFHIP_NN_GATHER");
     if (!(env && env[0] == '1')) {
@@ -284,7 +284,7 @@ static af_array conv2_filter_grad(const ArrayImpl &dy, const ArrayImpl &x, const
             check_k(afhip_k_conv2_nn_wgrad_rows(x.type, out_type, precision, o, x.data(), dy.data(), g.x.d,
                                                 g.w.d, g.y.d, g.st, g.pd, g.dl, ws->ptr, rb, stream()),
                     "convolve2_gradient_nn");
-            return out;
+            return out.release();
         }
     }
     const long long wsb = afhip_k_conv2_nn_wgrad_workspace(g.x.d, g.w.d, g.y.d, g.st, g.pd, g.dl);
@@ -293,7 +293,43 @@ static af_array conv2_filter_grad(const ArrayImpl &dy, const ArrayImpl &x, const
     check_k(afhip_k_conv2_nn_wgrad(x.type, out_type, precision, o, x.data(), dy.data(), g.x.d, g.w.d,
                                    g.y.d, g.st, g.pd, g.dl, ws ? ws->ptr : nullptr, wsb, stream()),
             "convolve2_gradient_nn");
-    return out;
+    return out.release();
+}
+
+// Scalar/scalar separable filters on a complex signal (src/api/c/convolve.cpp:57-68
+// with accT = cfloat / cdouble): y = x * (c * r), the filter product formed in
+// accT with each real product rounded before the sum, as std::complex does.
+template <typename R>
+static void cmul(const R *a, const R *b, R *o) {
+#pragma clang fp contract(off)
+    o[0] = a[0] * b[0] - a[1] * b[1];
+    o[1] = a[0] * b[1] + a[1] * b[0];
+}
+
+static af_array scale_complex(const ArrayImpl &sig, const std::shared_ptr<Buffer> &cb,
+                              const std::shared_ptr<Buffer> &rb, af_dtype at) {
+    double re, im;
+    if (at == c32) {
+        float c[2], r[2], p[2];
+        check_hip(hipMemcpyAsync(c, cb->ptr, 8, hipMemcpyDeviceToHost, stream()), "copy");
+        check_hip(hipMemcpyAsync(r, rb->ptr, 8, hipMemcpyDeviceToHost, stream()), "copy");
+        sync_stream();
+        cmul(c, r, p);
+        re = p[0];
+        im = p[1];
+    } else {
+        double c[2], r[2], p[2];
+        check_hip(hipMemcpyAsync(c, cb->ptr, 16, hipMemcpyDeviceToHost, stream()), "copy");
+        check_hip(hipMemcpyAsync(r, rb->ptr, 16, hipMemcpyDeviceToHost, stream()), "copy");
+        sync_stream();
+        cmul(c, r, p);
+        re = p[0];
+        im = p[1];
+    }
+    NewHandle o(make_handle(sig.type, sig.dims));
+    check_k(afhip_k_scale_complex(sig.type, o.data(), sig.data(), re, im, sig.elements(), stream()),
+            "convolve2_sep");
+    return o.release();
 }
 
 }  // namespace detail
@@ -399,6 +435,10 @@ AFAPI af_err af_convolve2_sep(af_array *out, const af_array col_filter, const af
         auto rb = cast_buffer(rf, at);
         if (cf.elements() == 1 && rf.elements() == 1) {
             // scalar/scalar: y = T(accT(x) * (c * r))
+            if (at == c32 || at == c64) {
+                *out = detail::scale_complex(s, cb, rb, at);
+                return AF_SUCCESS;
+            }
             if (at != f32 && at != f64) TYPE_ERROR(1, s.type);
             double scale;
             if (at == f32) {
@@ -414,10 +454,10 @@ AFAPI af_err af_convolve2_sep(af_array *out, const af_array col_filter, const af
                 sync_stream();
                 scale = c * r;
             }
-            af_array o = make_handle(s.type, s.dims);
-            check_k(afhip_k_scale(s.type, get(o).data(), s.data(), scale, s.elements(), stream()),
+            NewHandle o(make_handle(s.type, s.dims));
+            check_k(afhip_k_scale(s.type, o.data(), s.data(), scale, s.elements(), stream()),
                     "convolve2_sep");
-            *out = o;
+            *out = o.release();
             return AF_SUCCESS;
         }
         ARG_ASSERT(2, is_vector(cf.dims));
@@ -561,10 +601,10 @@ AFAPI af_err af_constant(af_array *arr, const double val, const unsigned ndims,
         ARG_ASSERT(2, ndims >= 1 && ndims <= 4);
         if (!dtype_known(type)) TYPE_ERROR(4, type);
         dim4 d(ndims, dims);
-        af_array o = make_handle(type, d);
+        NewHandle o(make_handle(type, d));
         if (d.elements() > 0)
-            check_k(afhip_k_fill(type, get(o).data(), val, d.elements(), stream()), "af_constant");
-        *arr = o;
+            check_k(afhip_k_fill(type, o.data(), val, d.elements(), stream()), "af_constant");
+        *arr = o.release();
     }
     CATCHALL;
     return AF_SUCCESS;
@@ -577,11 +617,11 @@ AFAPI af_err af_cast(af_array *out, const af_array in, const af_dtype type) {
         const ArrayImpl &a = get(in);
         if (!dtype_known(type)) TYPE_ERROR(2, type);
         if (a.type == type) return af_retain_array(out, in);
-        af_array o = make_handle(type, a.dims);
+        NewHandle o(make_handle(type, a.dims));
         if (a.elements() > 0)
-            check_k(afhip_k_cast(type, get(o).data(), a.type, a.data(), a.elements(), stream()),
+            check_k(afhip_k_cast(type, o.data(), a.type, a.data(), a.elements(), stream()),
                     "af_cast");
-        *out = o;
+        *out = o.release();
     }
     CATCHALL;
     return AF_SUCCESS;

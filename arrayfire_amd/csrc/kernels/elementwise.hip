@@ -271,3 +271,33 @@ extern "C" int afhip_k_scale(int dtype, void *out, const void *in, double scale,
 #undef CASE
     return launch_status();
 }
+
+namespace afhip {
+template <typename R>
+__global__ __launch_bounds__(256) void scale_complex_kernel(cplx<R> *__restrict__ out,
+                                                            const cplx<R> *__restrict__ in,
+                                                            cplx<R> scale, dim_t n) {
+    for (dim_t i = blockIdx.x * (dim_t)blockDim.x + threadIdx.x; i < n;
+         i += (dim_t)gridDim.x * blockDim.x)
+        out[i] = in[i] * scale;
+}
+}  // namespace afhip
+
+extern "C" int afhip_k_scale_complex(int dtype, void *out, const void *in, double re, double im,
+                                     long long n, void *stream) {
+    if (n <= 0) return K_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const unsigned g = grid_for(n);
+    switch (dtype) {
+        case DT_C32:
+            scale_complex_kernel<float><<<g, 256, 0, s>>>((cfloat *)out, (const cfloat *)in,
+                                                          cfloat((float)re, (float)im), n);
+            break;
+        case DT_C64:
+            scale_complex_kernel<double><<<g, 256, 0, s>>>((cdouble *)out, (const cdouble *)in,
+                                                           cdouble(re, im), n);
+            break;
+        default: return K_ERR_TYPE;
+    }
+    return launch_status();
+}

@@ -33,6 +33,7 @@ struct cplx {
     __host__ __device__ cplx(R r) : re(r), im(0) {}
     __host__ __device__ cplx(R r, R i) : re(r), im(i) {}
     __host__ __device__ cplx &operator+=(const cplx &o) {
+#pragma clang fp contract(off)
         re += o.re;
         im += o.im;
         return *this;
@@ -40,6 +41,8 @@ struct cplx {
 };
 template <typename R>
 __host__ __device__ inline cplx<R> operator*(const cplx<R> &a, const cplx<R> &b) {
+    // Each product rounded before the sum, as the reference CPU evaluates it.
+#pragma clang fp contract(off)
     return cplx<R>(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
 }
 typedef cplx<float> cfloat;

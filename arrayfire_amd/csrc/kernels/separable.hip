@@ -36,6 +36,8 @@ __global__ __launch_bounds__(256) void sep_pass(InT *__restrict__ out,
                                                 const InT *__restrict__ in,
                                                 const AccT *__restrict__ filt,
                                                 SepParams p) {
+    // Products are rounded before they are added, as on the reference CPU.
+#pragma clang fp contract(off)
     // Pass 0 reads the signal (dims sd) and writes the temp (dims td);
     // pass 1 reads the temp and writes the output (dims od).
     const dim_t *id = DIM == 0 ? p.sd : p.td;
@@ -506,6 +508,8 @@ extern "C" int afhip_k_convolve2_sep(int dtype, void *out, const long long *odim
     switch (dtype) {
         case DT_F32: return launch_two_pass<float, float>(out, in, cfilt, cflen, rfilt, rflen, p, tmp, s);
         case DT_F64: return launch_two_pass<double, double>(out, in, cfilt, cflen, rfilt, rflen, p, tmp, s);
+        case DT_C32: return launch_two_pass<cfloat, cfloat>(out, in, cfilt, cflen, rfilt, rflen, p, tmp, s);
+        case DT_C64: return launch_two_pass<cdouble, cdouble>(out, in, cfilt, cflen, rfilt, rflen, p, tmp, s);
         case DT_S32: return launch_two_pass<int32_t, float>(out, in, cfilt, cflen, rfilt, rflen, p, tmp, s);
         case DT_U32: return launch_two_pass<uint32_t, float>(out, in, cfilt, cflen, rfilt, rflen, p, tmp, s);
         case DT_S16: return launch_two_pass<int16_t, float>(out, in, cfilt, cflen, rfilt, rflen, p, tmp, s);

@@ -32,12 +32,17 @@ struct SpatialParams {
     int rank, expand;
 };
 
+// The product is rounded before it is added (no fused multiply-add), as the
+// reference's accum += AccT(s_val * f) is evaluated on the CPU: truncation to
+// an integer output then gives the reference's value bit for bit.
 template <typename InT, typename AccT>
 __device__ inline AccT mul_acc(InT s, AccT f) {
+#pragma clang fp contract(off)
     return AccT(s * f);
 }
 template <>
 __device__ inline float mul_acc<int8_t, float>(int8_t s, float f) {
+#pragma clang fp contract(off)
     return float(s) * f;
 }
 
@@ -51,6 +56,7 @@ __global__ __launch_bounds__(256) void conv_generic(InT *__restrict__ out,
                                                     const InT *__restrict__ in,
                                                     const AccT *__restrict__ filt,
                                                     SpatialParams p) {
+#pragma clang fp contract(off)
     dim_t ext[3] = {p.od[0], RANK > 1 ? p.od[1] : 1, RANK > 2 ? p.od[2] : 1};
     const dim_t nsp = ext[0] * ext[1] * ext[2];
     const dim_t total = nsp * p.batch[1] * p.batch[2] * p.batch[3];

@@ -6,6 +6,7 @@
 // table of problems and a cost-sorted list of (problem, tile) work items,
 // lets the autotuner pick the tile variant, and captures the single
 // persistent launch in a hipGraph; running the plan replays the graph.
+#include <cstdlib>
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -159,7 +160,13 @@ AFAPI af_err afhip_ragged_plan_create(afhip_ragged_plan *plan, af_array *outputs
                 h->p.items_v[v] = build_items(P, v, P.nitems[v]);
                 h->probs_v[v] = P.keep.back();
             }
+            // Persistent grid: 8 workgroups per CU. AFHIP_RAGGED_GRID caps it
+            // lower (each workgroup then walks more items of the list).
             P.grid = 256 * 8;
+            if (const char *ge = std::getenv("AFHIP_RAGGED_GRID")) {
+                const long long gv = std::atoll(ge);
+                if (gv > 0 && gv < P.grid) P.grid = (int)gv;
+            }
             const int choice = autotune_pick(key, 3, [&](int v) {
                 check_k(afhip_k_convolve2_ragged(v, h->probs_v[v]->ptr, P.items_v[v]->ptr, P.nitems[v],
                                                  (int)std::min<long long>(P.grid, std::max(1LL, P.nitems[v])),

@@ -253,6 +253,18 @@ af_array make_handle_shared(const ArrayImpl &src) {
     return register_impl(p);
 }
 
+void destroy_handle(af_array a) noexcept {
+    ArrayImpl *p = nullptr;
+    {
+        std::lock_guard<std::mutex> g(g_h_mutex);
+        auto *q = static_cast<ArrayImpl *>(a);
+        if (live().erase(q)) p = q;
+    }
+    delete p;
+}
+
+void *NewHandle::data() const { return ::afhip::get(h_).data(); }
+
 static void release_handle(af_array a) {
     ArrayImpl *p = nullptr;
     {

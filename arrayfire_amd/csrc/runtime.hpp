@@ -128,4 +128,30 @@ ArrayImpl *impl_of(af_array a);
 // Scratch device memory owned by a shared_ptr (returned to the pool).
 std::shared_ptr<Buffer> scratch(size_t bytes);
 
+// Releases a handle made by make_handle; unknown handles are ignored.
+void destroy_handle(af_array a) noexcept;
+
+// Owns a handle made by make_handle until release() hands it to the caller:
+// an exception thrown before then (a failed launch, a type error) releases
+// the handle and its storage instead of leaking them.
+class NewHandle {
+   public:
+    explicit NewHandle(af_array h) : h_(h) {}
+    NewHandle(const NewHandle &) = delete;
+    NewHandle &operator=(const NewHandle &) = delete;
+    ~NewHandle() {
+        if (h_) destroy_handle(h_);
+    }
+    af_array get() const { return h_; }
+    void *data() const;
+    af_array release() {
+        af_array h = h_;
+        h_ = nullptr;
+        return h;
+    }
+
+   private:
+    af_array h_;
+};
+
 }  // namespace afhip

@@ -64,6 +64,14 @@ AFHIP_KAPI int afhip_k_fill(int dtype, void *out, double value, long long n,
 AFHIP_KAPI int afhip_k_scale(int dtype, void *out, const void *in,
                              double scale, long long n, void *stream);
 
+/* y = x * (re + i im) elementwise for complex x (c32 or c64, the product
+ * formed in the signal's precision with each real product rounded before the
+ * sum): the scalar/scalar separable case for complex signals
+ * (src/api/c/convolve.cpp:57-68 with accT = cfloat/cdouble). */
+AFHIP_KAPI int afhip_k_scale_complex(int dtype, void *out, const void *in,
+                                     double re, double im, long long n,
+                                     void *stream);
+
 /* Precision codes below are afhip_precision values (include/af/hip.h):
  * 0 default (by in_type), 1 BF16 MFMA, 2 FP16 MFMA, 3 split-BF16, 4 FP32.
  * in_type is f32 (0), f16 (12) or bfloat16 (64); out_type f32, or the 16-bit

@@ -64,7 +64,7 @@ def _lib():
         lib = ctypes.CDLL(path)
         p = ctypes.c_void_p
         i64 = ctypes.c_int64
-        for suf in ("f32", "f64", "s32", "u8"):
+        for suf in _ALL_SUFFIXES:
             f = getattr(lib, f"orc_convolve_nd_{suf}")
             f.argtypes = [p, p, p, p, p, p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
             f.restype = None
@@ -83,10 +83,28 @@ def _lib():
 
 
 # ----------------------------------------------------------------- helpers
+# signal dtype -> (C instantiation, accumulator dtype), src/api/c/convolve.cpp:155-209.
+# b8 (numpy bool) runs as the reference's `char`: outputs are returned as int8
+# because a sum of several taps need not be 0 or 1.
 _SUFFIX = {np.dtype(np.float32): ("f32", np.float32),
            np.dtype(np.float64): ("f64", np.float64),
+           np.dtype(np.complex64): ("c32", np.complex64),
+           np.dtype(np.complex128): ("c64", np.complex128),
            np.dtype(np.int32): ("s32", np.float32),
-           np.dtype(np.uint8): ("u8", np.float32)}
+           np.dtype(np.uint32): ("u32", np.float32),
+           np.dtype(np.int16): ("s16", np.float32),
+           np.dtype(np.uint16): ("u16", np.float32),
+           np.dtype(np.int64): ("s64", np.float32),
+           np.dtype(np.uint64): ("u64", np.float32),
+           np.dtype(np.uint8): ("u8", np.float32),
+           np.dtype(np.int8): ("s8", np.float32),
+           np.dtype(np.bool_): ("b8", np.float32)}
+_ALL_SUFFIXES = tuple(v[0] for v in _SUFFIX.values())
+
+
+def _storage(dt):
+    """numpy dtype the C instantiation reads and writes (bool -> int8)."""
+    return np.dtype(np.int8) if np.dtype(dt) == np.bool_ else np.dtype(dt)
 
 
 def as4(a) -> np.ndarray:
@@ -112,9 +130,12 @@ def _ptr(a: np.ndarray):
 
 
 def rel_l2(a, b) -> float:
-    """||a - b||_2 / ||b||_2 computed in float64."""
-    a = np.asarray(a, dtype=np.float64).ravel()
-    b = np.asarray(b, dtype=np.float64).ravel()
+    """||a - b||_2 / ||b||_2 computed in float64 (complex128 for complex
+    operands, so both parts count)."""
+    a, b = np.asarray(a), np.asarray(b)
+    dt = np.complex128 if (np.iscomplexobj(a) or np.iscomplexobj(b)) else np.float64
+    a = a.astype(dt).ravel()
+    b = b.astype(dt).ravel()
     nb = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))
 
@@ -212,9 +233,10 @@ def convolve(signal, filt, rank: int, expand: bool) -> np.ndarray:
     if kind in (AF_BATCH_DIFF, AF_BATCH_UNSUPPORTED):
         raise ValueError("unsupported batch kind")
     od = conv_out_dims(sd, fd, kind, rank, expand)
-    xs = _flat(x, x.dtype)
+    st = _storage(x.dtype)
+    xs = _flat(x, st)
     fs = _flat(as4(filt), acct)
-    out = np.zeros(int(np.prod(od)), dtype=x.dtype)
+    out = np.zeros(int(np.prod(od)), dtype=st)
     getattr(_lib(), f"orc_convolve_nd_{suf}")(
         _ptr(out), _ptr(_dimbuf(od)), _ptr(xs), _ptr(_dimbuf(sd)), _ptr(fs),
         _ptr(_dimbuf(fd)), kind, rank, int(bool(expand)))
@@ -241,14 +263,15 @@ def convolve2_sep(col_filter, row_filter, signal, expand: bool) -> np.ndarray:
     rf = np.asarray(row_filter, dtype=acct).ravel(order="F")
     if cf.size == 1 and rf.size == 1:
         prod = acct(cf[0]) * acct(rf[0])
-        return (x.astype(acct) * prod).astype(x.dtype)
+        return (x.astype(acct) * prod).astype(_storage(x.dtype))
     sd = dims4(x)
     od = list(sd)
     if expand:
         od[0] += cf.size - 1
         od[1] += rf.size - 1
-    out = np.zeros(int(np.prod(od)), dtype=x.dtype)
-    xs = _flat(x, x.dtype)
+    st = _storage(x.dtype)
+    out = np.zeros(int(np.prod(od)), dtype=st)
+    xs = _flat(x, st)
     rc = getattr(_lib(), f"orc_convolve2_sep_{suf}")(
         _ptr(out), _ptr(_dimbuf(od)), _ptr(xs), _ptr(_dimbuf(sd)),
         _ptr(np.ascontiguousarray(cf)), cf.size, _ptr(np.ascontiguousarray(rf)),

@@ -14,8 +14,8 @@
  *   unwrap_dim (im2col, d = 0) ............. src/backend/cpu/kernel/unwrap.hpp:20-80
  *   wrap_dim_dilated (col2im, d = 0) ....... src/backend/cpu/kernel/wrap.hpp:83-145
  *
- * Every function is instantiated for several (InT, AccT) pairs through a
- * macro: (float,float), (double,double) and (int32,float), (uint8,float).
+ * Every function is instantiated for the reference's (InT, AccT) pairs
+ * through a macro: every af_dtype the spatial path accepts.
  * The double instantiation of the spatial loops is also the FP64 brute-force
  * definition of the spatial convolution (it is the textbook sum with no
  * reordering).
@@ -26,6 +26,7 @@
  * Build: oracle/Makefile (gcc -O3 -ffp-contract=off so that a*b+c is
  * evaluated as two rounded operations, as the reference source spells it).
  */
+#include <complex.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -233,10 +234,25 @@ int orc_convolve2_sep_##SUF(InT *optr, const dim_t *oD, const InT *iptr,        
     return 0;                                                                   \
 }
 
+/* The (InT, AccT) pairs of src/api/c/convolve.cpp:155-209 (c32 -> c32,
+ * c64 -> c64, f32 -> f32, f64 -> f64, every integer type and b8 -> float).
+ * b8 is the reference's `char`; the complex types are C99's, whose product
+ * (ac - bd, ad + bc) with -ffp-contract=off rounds as std::complex does. */
+typedef float _Complex cf32_t;
+typedef double _Complex cf64_t;
 DEFINE_SPATIAL(f32, float, float)
 DEFINE_SPATIAL(f64, double, double)
+DEFINE_SPATIAL(c32, cf32_t, cf32_t)
+DEFINE_SPATIAL(c64, cf64_t, cf64_t)
 DEFINE_SPATIAL(s32, int32_t, float)
+DEFINE_SPATIAL(u32, uint32_t, float)
+DEFINE_SPATIAL(s16, int16_t, float)
+DEFINE_SPATIAL(u16, uint16_t, float)
+DEFINE_SPATIAL(s64, int64_t, float)
+DEFINE_SPATIAL(u64, uint64_t, float)
 DEFINE_SPATIAL(u8, uint8_t, float)
+DEFINE_SPATIAL(s8, int8_t, float)
+DEFINE_SPATIAL(b8, char, float)
 
 /* im2col with d = 0 (retCols = false), src/backend/cpu/kernel/unwrap.hpp:20-80
  * and the output shape of src/backend/cpu/unwrap.cpp:23-42:

@@ -53,7 +53,8 @@ def test_known_answer_3d(golden):
 
 
 def _scipy_conv(x, w, expand):
-    full = ss.convolve(x.astype(np.float64), w.astype(np.float64), mode="full", method="direct")
+    dt = np.complex128 if (np.iscomplexobj(x) or np.iscomplexobj(w)) else np.float64
+    full = ss.convolve(x.astype(dt), w.astype(dt), mode="full", method="direct")
     if expand:
         return full
     sl = tuple(slice(f // 2, f // 2 + s) for s, f in zip(x.shape, w.shape))
@@ -215,3 +216,113 @@ def test_gaussian_kernel_closed_form():
     r = np.arange(5) - 2.0
     e = np.exp(-0.5 * ((r[:, None] / 1.25) ** 2 + (r[None, :] / 1.25) ** 2))
     np.testing.assert_allclose(d, e / e.sum(), rtol=1e-14)
+
+
+# ------------------------------------------- integer and complex instantiations
+ALL_INT = [np.int32, np.uint32, np.int16, np.uint16, np.int64, np.uint64, np.uint8, np.int8,
+           np.bool_]
+
+
+def _raw(a):
+    a = np.asarray(a)
+    return a.view(np.int8) if a.dtype == np.bool_ else a
+
+
+@pytest.mark.parametrize("dt", ALL_INT + [np.complex64, np.complex128],
+                         ids=lambda d: np.dtype(d).name)
+def test_every_dtype_matches_scipy_on_integer_data(dt):
+    """Every instantiation computes the textbook convolution: integer-valued
+    data and filters give exact sums, so scipy's FP64 result is the answer."""
+    rng = np.random.default_rng(3)
+    for sd, fd, rank in (((23,), (4,), 1), ((13, 11), (3, 4), 2), ((7, 6, 5), (2, 3, 2), 3)):
+        if np.dtype(dt) == np.bool_:
+            x = rng.integers(0, 2, sd).astype(np.bool_)
+        elif np.issubdtype(np.dtype(dt), np.complexfloating):
+            x = (rng.integers(0, 4, sd) + 1j * rng.integers(0, 4, sd)).astype(dt)
+        else:
+            x = rng.integers(0, 4, sd).astype(dt)
+        w = rng.integers(0, 3, fd).astype(np.float32)
+        for expand in (False, True):
+            got = oracle.convolve(x, w, rank, expand)
+            ref = _scipy_conv(x.astype(np.complex128), w, expand)
+            assert got.dtype == (np.int8 if np.dtype(dt) == np.bool_ else np.dtype(dt))
+            np.testing.assert_array_equal(got.reshape(ref.shape).astype(np.complex128), ref)
+
+
+@pytest.mark.parametrize("dt", [np.int32, np.int16, np.int64, np.int8], ids=lambda d: np.dtype(d).name)
+def test_integer_output_truncates_toward_zero(dt):
+    """InT(accum) truncates (kernel/convolve.hpp:35): x * 0.5 on odd negative
+    and positive values rounds toward zero, not to the nearest integer nor
+    down."""
+    x = np.array([-7, -5, -3, -1, 1, 3, 5, 7], dtype=dt)
+    got = oracle.convolve(x, np.float32([0.5]), 1, False).ravel()
+    np.testing.assert_array_equal(got, np.trunc(x.astype(np.float64) * 0.5).astype(dt))
+
+
+@pytest.mark.parametrize("dt", [np.int32, np.uint8, np.int16, np.uint64, np.int8, np.bool_],
+                         ids=lambda d: np.dtype(d).name)
+def test_separable_taps_truncate_to_signal_type(dt):
+    """convolve2_separable casts each tap to InT (kernel/convolve.hpp:233):
+    fractional taps act as their truncations, so the result equals the exact
+    integer 2-D convolution with the outer product of the truncated taps."""
+    rng = np.random.default_rng(11)
+    unsigned = np.dtype(dt).kind in "ub"
+    x = (rng.integers(0, 2, (17, 14)) if np.dtype(dt).itemsize == 1
+         else rng.integers(0, 9, (17, 14))).astype(dt)
+    cf = np.float32([2.7, 1.2, 0.6, 2.2]) if unsigned else np.float32([2.7, -1.5, 0.6, -2.2])
+    rf = np.float32([1.9, 2.5, 1.1]) if unsigned else np.float32([-1.9, 2.5, 1.1])
+    ct, rt = np.trunc(cf), np.trunc(rf)
+    for expand in (False, True):
+        got = oracle.convolve2_sep(cf, rf, x, expand)
+        ref = _scipy_conv(x.astype(np.float64), np.outer(ct, rt), expand)
+        np.testing.assert_array_equal(_raw(got)[:, :, 0, 0], ref)
+        # the untruncated taps would give a different answer (the pin bites)
+        assert not np.array_equal(_scipy_conv(x.astype(np.float64), np.outer(cf, rf), expand), ref)
+
+
+def _brute_2d_float_filter(x, w, expand, dt):
+    """Pure-Python loop in the reference's order (kernel/convolve.hpp:39-75)
+    with float32 products rounded before each float32 add, then truncation."""
+    S0, S1 = x.shape
+    F0, F1 = w.shape
+    O0, O1 = (S0 + F0 - 1, S1 + F1 - 1) if expand else (S0, S1)
+    i0, j0 = (0, 0) if expand else (F0 // 2, F1 // 2)
+    out = np.zeros((O0, O1), dtype=np.int64)
+    for oj in range(O1):
+        for oi in range(O0):
+            acc = np.float32(0)
+            for wj in range(F1):
+                for wi in range(F0):
+                    a, b = oi + i0 - wi, oj + j0 - wj
+                    s = x[a, b] if (0 <= a < S0 and 0 <= b < S1) else 0
+                    acc = np.float32(acc + np.float32(np.float32(int(s)) * w[wi, wj]))
+            out[oi, oj] = int(acc)   # truncation toward zero
+    return out
+
+
+@pytest.mark.parametrize("dt", ALL_INT, ids=lambda d: np.dtype(d).name)
+def test_integer_instantiations_brute_force(dt):
+    """Fractional float filters on integer signals: every instantiation
+    against a pure-Python brute force of the same rounding points."""
+    rng = np.random.default_rng(7)
+    unsigned = np.dtype(dt).kind in "ub"
+    x = (rng.integers(0, 2, (9, 8)) if np.dtype(dt) == np.bool_ else rng.integers(0, 7, (9, 8)))
+    x = x.astype(dt)
+    w = rng.uniform(0.0 if unsigned else -1.0, 1.0, (3, 4)).astype(np.float32)
+    for expand in (False, True):
+        got = _raw(oracle.convolve(x, w, 2, expand))[:, :, 0, 0]
+        np.testing.assert_array_equal(got.astype(np.int64), _brute_2d_float_filter(x, w, expand, dt))
+
+
+def test_complex_instantiation_matches_numpy_fp64():
+    """The complex instantiations against an FP64 evaluation of the complex
+    sum (scipy on complex128)."""
+    rng = np.random.default_rng(5)
+    x = rng.random((12, 9)) + 1j * rng.random((12, 9))
+    w = rng.uniform(-1, 1, (4, 3)) + 1j * rng.uniform(-1, 1, (4, 3))
+    for expand in (False, True):
+        ref = _scipy_conv(x, w, expand)
+        g64 = oracle.convolve(x, w, 2, expand)[:, :, 0, 0]
+        g32 = oracle.convolve(x.astype(np.complex64), w.astype(np.complex64), 2, expand)[:, :, 0, 0]
+        assert oracle.rel_l2(g64, ref) <= 1e-14
+        assert oracle.rel_l2(g32, ref) <= 1e-6

@@ -72,3 +72,59 @@ def test_ragged_rejects_1d_signal(af, monkeypatch):
 def test_ragged_empty(af, monkeypatch):
     monkeypatch.setenv("AFHIP_AUTOTUNE_CACHE", "off")
     assert af.convolve2_ragged([], [], []) == []
+
+
+def _check_all(sig, flt, modes, outs, workers=16):
+    """Every output against its own af_convolve2 call restated by the oracle
+    (the oracle is C behind ctypes, which releases the GIL: a thread pool)."""
+    import concurrent.futures as cf
+
+    def one(i):
+        ref = oracle.convolve_dispatch(sig[i], flt[i], 2, modes[i] == 1)[:, :, 0, 0]
+        got = outs[i]
+        assert got.shape == ref.shape, (i, got.shape, ref.shape)
+        return oracle.rel_l2(got, ref)
+
+    with cf.ThreadPoolExecutor(workers) as ex:
+        errs = list(ex.map(one, range(len(sig))))
+    worst = int(np.argmax(errs))
+    assert errs[worst] <= 1e-5, (worst, errs[worst])
+
+
+def test_ragged_full_problem_list(af, monkeypatch):
+    """The committed 2000-problem list (image sides log-uniform in [64, 1024],
+    odd filter sides 3..15, mixed DEFAULT/EXPAND): tens of thousands of
+    (problem, tile) items on a grid of 2048 persistent workgroups, so every
+    workgroup walks many items and reuses its LDS tile and filter across
+    problems. Every output is checked, and a replay gives the same bits."""
+    import json
+
+    monkeypatch.setenv("AFHIP_AUTOTUNE_CACHE", "off")
+    here = os.path.dirname(os.path.abspath(__file__))
+    with open(os.path.join(here, "golden", "ragged_problems.json")) as f:
+        probs = json.load(f)["problems"]
+    rng = np.random.default_rng(20261202 + 4)
+    sig = [rng.random((w, h), dtype=np.float32) for w, h, _, _, _ in probs]
+    flt = [rng.uniform(-1, 1, (fw, fh)).astype(np.float32) for _, _, fw, fh, _ in probs]
+    modes = [m for *_, m in probs]
+    plan = af.RaggedPlan(sig, flt, modes)
+    plan.run()
+    outs = [o.to_numpy()[:, :, 0, 0] for o in plan.outputs]
+    print(f"ragged full list: {len(probs)} problems, variant {plan.variant}", flush=True)
+    _check_all(sig, flt, modes, outs)
+    plan.run()
+    for i in (0, 777, 1999):
+        np.testing.assert_array_equal(plan.outputs[i].to_numpy()[:, :, 0, 0], outs[i])
+
+
+@pytest.mark.parametrize("grid", [1, 7, 64])
+def test_ragged_small_grid_loops_over_items(af, rng, monkeypatch, grid):
+    """AFHIP_RAGGED_GRID caps the persistent grid: with 1, 7 or 64 workgroups
+    for hundreds of items, each workgroup runs many items back to back
+    (problems of different sizes, filters and modes in one workgroup)."""
+    monkeypatch.setenv("AFHIP_AUTOTUNE_CACHE", "off")
+    monkeypatch.setenv("AFHIP_RAGGED_GRID", str(grid))
+    sig, flt, modes = _problems(rng, 80, 2, 200)
+    plan = af.RaggedPlan(sig, flt, modes)
+    plan.run()
+    _check_all(sig, flt, modes, [o.to_numpy()[:, :, 0, 0] for o in plan.outputs])

@@ -72,11 +72,16 @@ def test_f64_and_int(af, rng):
     cf, rf = rng.uniform(-1, 1, 5), rng.uniform(-1, 1, 7)
     got = af.convolve_sep(cf, rf, x, 0).to_numpy()
     assert oracle.rel_l2(got, oracle.convolve2_sep(cf, rf, x, False)) <= 1e-13
+    # int32 signal: taps truncate to int (kernel/convolve.hpp:233), so use
+    # taps beyond +-1; every product and sum is then an exact integer
     xi = rng.integers(0, 100, (40, 30)).astype(np.int32)
-    got = af.convolve_sep(cf.astype(np.float32), rf.astype(np.float32), xi, 1).to_numpy()
-    ref = oracle.convolve2_sep(cf.astype(np.float32), rf.astype(np.float32), xi, True)
+    cfi = rng.uniform(-3.9, 3.9, 5).astype(np.float32)
+    rfi = rng.uniform(-3.9, 3.9, 7).astype(np.float32)
+    got = af.convolve_sep(cfi, rfi, xi, 1).to_numpy()
+    ref = oracle.convolve2_sep(cfi, rfi, xi, True)
     assert got.dtype == np.int32 and got.shape == ref.shape
-    assert np.max(np.abs(got.astype(np.int64) - ref)) <= 1
+    assert np.any(ref != 0)
+    np.testing.assert_array_equal(got, ref)
 
 
 def test_gaussian_17_sigma3(af, rng):
