@@ -79,6 +79,7 @@ EXPORTED = [
     "afhip_ragged_plan_release", "afhip_convolve2_ragged", "afhip_k_ragged_tile",
     "afhip_k_convolve2_ragged", "afhip_k_conv2_nn_patch_pack", "afhip_k_conv2_nn_patch_run",
     "afhip_k_conv2_nn_wgrad_rows_partial", "afhip_k_conv2_nn_wgrad_rows_reduce",
+    "afhip_k_fft_convolve_workspace", "afhip_k_fft_convolve",
 ]
 
 

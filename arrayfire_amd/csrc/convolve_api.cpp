@@ -5,6 +5,7 @@
 // The backend layer ("detail", src/backend/cpu/convolve.cpp:38-243) computes
 // output dims, allocates from the stream-ordered pool and queues the gfx950
 // kernels through the kernel C ABI (include/afhip_kernels.h).
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <vector>
@@ -296,6 +297,45 @@ static af_array conv2_filter_grad(const ArrayImpl &dy, const ArrayImpl &x, const
     return out.release();
 }
 
+// Frequency-domain convolution (src/backend/cpu/fftconvolve.cpp:42-203 and
+// the fallback of src/api/c/fftconvolve.cpp:52-121). The batched path takes
+// the filter converted to the signal's type (castArray<T>: fractional taps on
+// an integer signal truncate); the fallback (DIFF batches, complex signals)
+// converts it straight to the complex compute type.
+static af_array fft_convolve(const ArrayImpl &sig, const ArrayImpl &filt, BatchKind kind, int rank,
+                             bool expand) {
+    const bool fallback = kind == BATCH_DIFF || sig.type == c32 || sig.type == c64;
+    std::shared_ptr<Buffer> fb;
+    af_dtype ft;
+    if (fallback) {
+        ft = (filt.type == f16 || filt.type == afhip_bf16) ? f32 : filt.type;
+        fb = cast_buffer(filt, ft);
+    } else {
+        ft = sig.type;
+        fb = cast_buffer(filt, ft);
+    }
+    dim4 od;
+    if (kind == BATCH_DIFF) {
+        // base dims as the batched kinds; dims past the rank: max(s, f)
+        // (odims of src/api/c/fftconvolve.cpp:72-97)
+        for (int d = 0; d < AF_MAX_DIMS; ++d)
+            od[d] = d < rank ? (expand ? sig.dims[d] + filt.dims[d] - 1 : sig.dims[d])
+                             : std::max(sig.dims[d], filt.dims[d]);
+    } else {
+        od = conv_out_dims(sig.dims, filt.dims, kind, rank, expand);
+    }
+    NewHandle out(make_handle(sig.type, od));
+    if (od.elements() == 0) return out.release();
+    const long long need =
+        afhip_k_fft_convolve_workspace(sig.type, sig.dims.d, filt.dims.d, od.d, (int)kind, rank);
+    if (need < 0) AFHIP_THROW(AF_ERR_SIZE, "fft_convolve: a padded axis exceeds 2^24 elements");
+    auto ws = scratch((size_t)need);
+    check_k(afhip_k_fft_convolve(sig.type, out.data(), od.d, sig.data(), sig.dims.d, ft, fb->ptr,
+                                 filt.dims.d, (int)kind, rank, expand ? 1 : 0, ws->ptr, need, stream()),
+            "fft_convolve");
+    return out.release();
+}
+
 // Scalar/scalar separable filters on a complex signal (src/api/c/convolve.cpp:57-68
 // with accT = cfloat / cdouble): y = x * (c * r), the filter product formed in
 // accT with each real product rounded before the sum, as std::complex does.
@@ -354,23 +394,52 @@ static af_err convolve_common(af_array *out, const af_array signal, const af_arr
     return AF_SUCCESS;
 }
 
+// src/api/c/fftconvolve.cpp:162-219: batch kind, dtype dispatch (every
+// dtype but f16), then the batched path or, for DIFF batches and complex
+// signals, the fallback (:52-121).
+static af_err fft_convolve_common(af_array *out, const af_array signal, const af_array filter,
+                                  const bool expand, const int rank) {
+    try {
+        const ArrayImpl &s = get(signal);
+        const ArrayImpl &f = get(filter);
+        ARG_ASSERT(0, out != nullptr);
+        const BatchKind kind = identify_batch_kind(rank, s.dims, f.dims);
+        ARG_ASSERT(1, kind != BATCH_UNSUPPORTED);
+        (void)acc_type(s.type);  // TYPE_ERROR(1, stype) for f16 / bf16
+        *out = detail::fft_convolve(s, f, kind, rank, expand);
+    }
+    CATCHALL;
+    return AF_SUCCESS;
+}
+
 }  // namespace afhip
 
 using namespace afhip;
 
 extern "C" {
 
-AFAPI af_err af_fft_convolve1(af_array *, const af_array, const af_array, const af_conv_mode) {
-    set_last_error("frequency-domain convolution is not provided by this library");
-    return AF_ERR_NOT_SUPPORTED;
+// src/api/c/fftconvolve.cpp:247-264
+AFAPI af_err af_fft_convolve1(af_array *out, const af_array signal, const af_array filter,
+                              const af_conv_mode mode) {
+    return fft_convolve_common(out, signal, filter, mode == AF_CONV_EXPAND, 1);
 }
-AFAPI af_err af_fft_convolve2(af_array *, const af_array, const af_array, const af_conv_mode) {
-    set_last_error("frequency-domain convolution is not provided by this library");
-    return AF_ERR_NOT_SUPPORTED;
+AFAPI af_err af_fft_convolve2(af_array *out, const af_array signal, const af_array filter,
+                              const af_conv_mode mode) {
+    try {
+        if (get(signal).dims.ndims() < 2 && get(filter).dims.ndims() < 2)
+            return fft_convolve_common(out, signal, filter, mode == AF_CONV_EXPAND, 1);
+        return fft_convolve_common(out, signal, filter, mode == AF_CONV_EXPAND, 2);
+    }
+    CATCHALL;
 }
-AFAPI af_err af_fft_convolve3(af_array *, const af_array, const af_array, const af_conv_mode) {
-    set_last_error("frequency-domain convolution is not provided by this library");
-    return AF_ERR_NOT_SUPPORTED;
+AFAPI af_err af_fft_convolve3(af_array *out, const af_array signal, const af_array filter,
+                              const af_conv_mode mode) {
+    try {
+        if (get(signal).dims.ndims() < 3 && get(filter).dims.ndims() < 3)
+            return fft_convolve_common(out, signal, filter, mode == AF_CONV_EXPAND, 2);
+        return fft_convolve_common(out, signal, filter, mode == AF_CONV_EXPAND, 3);
+    }
+    CATCHALL;
 }
 
 // src/api/c/convolve.cpp:217-226

@@ -50,6 +50,26 @@ AFHIP_KAPI int afhip_k_convolve2_sep(int dtype, void *out, const long long *odim
                                      const void *rfilt, long long rflen,
                                      int expand, void *tmp, void *stream);
 
+/* detail::fftconvolve<T> (src/backend/cpu/fftconvolve.hpp:16-19,
+ * src/backend/cpu/fftconvolve.cpp:42-203) and the DIFF / complex fallback of
+ * src/api/c/fftconvolve.cpp:52-121: frequency-domain convolution of rank 1-3.
+ * dtype is the signal's af_dtype; ftype the filter's (the batched path takes
+ * the filter already converted to the signal's type, as castArray<T> does;
+ * the fallback any type). kind is the AF_BATCH_KIND (NONE=1 .. DIFF=5); odims
+ * the reference's output dims (for DIFF, max(s, f) past the rank). The
+ * workspace holds the padded spectra: query its size first (-1: a padded
+ * axis above 2^24). Integer outputs are rounded to nearest (batched kinds) or
+ * truncated (DIFF), as the reference does. */
+AFHIP_KAPI long long afhip_k_fft_convolve_workspace(int dtype, const long long *sdims,
+                                                    const long long *fdims,
+                                                    const long long *odims, int kind,
+                                                    int rank);
+AFHIP_KAPI int afhip_k_fft_convolve(int dtype, void *out, const long long *odims,
+                                    const void *in, const long long *sdims, int ftype,
+                                    const void *filt, const long long *fdims, int kind,
+                                    int rank, int expand, void *workspace,
+                                    long long workspace_bytes, void *stream);
+
 /* Element type conversion between af_dtype values (and 64 = bfloat16) over n
  * contiguous elements. */
 AFHIP_KAPI int afhip_k_cast(int out_type, void *out, int in_type,

@@ -252,6 +252,65 @@ def convolve_dispatch(signal, filt, rank: int, expand: bool) -> np.ndarray:
     return convolve(signal, filt, rank, expand)
 
 
+def _round_half_away(v):
+    """roundf: halves away from zero."""
+    return np.sign(v) * np.floor(np.abs(v) + 0.5)
+
+
+def fft_convolve(signal, filt, rank: int, expand: bool) -> np.ndarray:
+    """af_fft_convolve{rank} semantics, src/api/c/fftconvolve.cpp:123-219 with
+    src/backend/cpu/fftconvolve.cpp:42-203 and the DIFF / complex fallback
+    (src/api/c/fftconvolve.cpp:52-121), evaluated exactly (FP64 / complex128
+    direct sums through this module's spatial restatement) and then converted
+    as the reference converts:
+
+    * batched kinds (NONE, LHS, RHS, SAME) of a real signal: the filter is
+      first cast to the signal's type (castArray<T>, so fractional taps on an
+      integer signal truncate); integer outputs are rounded half away from
+      zero (roundf, kernel/fftconvolve.hpp:186-218), float outputs cast;
+    * DIFF batches and complex signals: the filter is cast to the complex
+      compute type; the output is a plain cast of the real part (integers
+      truncate) or the complex value.
+
+    Output dims: the spatial rules for batched kinds; for DIFF, the base dims
+    of those rules and max(s, f) past the rank (broadcast pairing).  The FFT
+    itself only adds rounding of the order of the compute type's epsilon,
+    which the tests bound."""
+    x = as4(signal)
+    f = as4(filt)
+    sd, fd = dims4(x), dims4(f)
+    kind = identify_batch_kind(rank, sd, fd)
+    if kind == AF_BATCH_UNSUPPORTED:
+        raise ValueError("unsupported batch kind")
+    cplx = np.iscomplexobj(x)
+    fallback = kind == AF_BATCH_DIFF or cplx
+    st = _storage(x.dtype)
+    if not fallback:
+        f = f.astype(st)                       # castArray<T>(filter)
+    wide = np.complex128 if (cplx or np.iscomplexobj(f)) else np.float64
+    if kind == AF_BATCH_DIFF:
+        od = tuple((sd[d] + fd[d] - 1 if expand else sd[d]) if d < rank else max(sd[d], fd[d])
+                   for d in range(4))
+    else:
+        od = conv_out_dims(sd, fd, kind, rank, expand)
+    out = np.zeros(od, dtype=wide)
+    xs, fs = x.astype(st).astype(wide), f.astype(wide)
+    for ob in np.ndindex(*od[rank:]):
+        si = tuple(0 if sd[rank + i] == 1 else ob[i] for i in range(len(ob)))
+        fi = tuple(0 if fd[rank + i] == 1 else ob[i] for i in range(len(ob)))
+        sl = (slice(None),) * rank
+        xs1 = np.ascontiguousarray(xs[sl + si].reshape(sd[:rank] + (1,) * (4 - rank)))
+        fs1 = np.ascontiguousarray(fs[sl + fi].reshape(fd[:rank] + (1,) * (4 - rank)))
+        r = convolve(xs1, fs1, rank, expand)
+        out[sl + ob] = r.reshape(r.shape[:rank])
+    if cplx:
+        return out.astype(x.dtype)
+    v = out.real
+    if np.issubdtype(st, np.integer):
+        v = np.trunc(v) if kind == AF_BATCH_DIFF else _round_half_away(v)
+    return v.astype(st)
+
+
 def convolve2_sep(col_filter, row_filter, signal, expand: bool) -> np.ndarray:
     """Separable 2-D convolution, src/backend/cpu/convolve.cpp:68-99 and
     src/backend/cpu/kernel/convolve.hpp:213-289 (InT-typed temp).

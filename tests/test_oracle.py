@@ -326,3 +326,50 @@ def test_complex_instantiation_matches_numpy_fp64():
         g32 = oracle.convolve(x.astype(np.complex64), w.astype(np.complex64), 2, expand)[:, :, 0, 0]
         assert oracle.rel_l2(g64, ref) <= 1e-14
         assert oracle.rel_l2(g32, ref) <= 1e-6
+
+
+# ------------------------------------------------------ frequency domain
+def _np_fft_linear(x, w, expand):
+    """numpy FFT convolution with nextpow2(s + f - 1) padding per axis, then
+    the crop at f // 2 (DEFAULT) or 0 (EXPAND)."""
+    shp = [1 << int(np.ceil(np.log2(a + b - 1))) for a, b in zip(x.shape, w.shape)]
+    y = np.fft.ifftn(np.fft.fftn(x, shp) * np.fft.fftn(w, shp))
+    full = tuple(slice(0, a + b - 1) for a, b in zip(x.shape, w.shape))
+    y = y[full]
+    if not expand:
+        y = y[tuple(slice(b // 2, b // 2 + a) for a, b in zip(x.shape, w.shape))]
+    return y
+
+
+@pytest.mark.parametrize("sd,fd", [((37,), (9,)), ((20, 13), (7, 4)), ((9, 8, 7), (3, 4, 2))])
+def test_fft_convolve_is_padded_fft_product(sd, fd):
+    """The restated frequency-domain result equals numpy's FFT convolution
+    with the reference's nextpow2 padding and crop offsets (no wrap-around,
+    offset f // 2 in DEFAULT mode)."""
+    rng = np.random.default_rng(1)
+    x, w = rng.standard_normal(sd), rng.standard_normal(fd)
+    for expand in (False, True):
+        got = oracle.fft_convolve(x, w, len(sd), expand).reshape(
+            tuple(a + b - 1 for a, b in zip(sd, fd)) if expand else sd)
+        np.testing.assert_allclose(got, _np_fft_linear(x, w, expand).real, atol=1e-12)
+
+
+def test_fft_convolve_output_rules():
+    """Integer outputs: rounded half away from zero on the batched path,
+    truncated on the DIFF path; filter taps truncated to the signal type on
+    the batched path only."""
+    x = np.array([3, -3, 5], np.int32)
+    # batched: taps [0.5, 1.5] -> int [0, 1]: y = x shifted, exact integers
+    got = oracle.fft_convolve(x, np.float32([0.5, 1.5]), 1, True).ravel()
+    np.testing.assert_array_equal(got, ss.convolve(x, [0, 1]))
+    # DIFF (signal [3,1,2], filter [2,1,1,2]): taps stay fractional, truncation
+    xs = np.array([[[1], [2]], [[-3], [4]], [[5], [-6]]], np.int32).reshape(3, 1, 2, 1)
+    fs = np.array([0.5, 0.25], np.float32).reshape(2, 1, 1, 1) * np.ones((1, 1, 1, 2), np.float32)
+    assert oracle.identify_batch_kind(1, xs.shape, fs.shape) == oracle.AF_BATCH_DIFF
+    got = oracle.fft_convolve(xs, fs, 1, True)
+    assert got.shape == (4, 1, 2, 2)
+    exact = ss.convolve(xs[:, 0, 1, 0].astype(np.float64), [0.5, 0.25])
+    np.testing.assert_array_equal(got[:, 0, 1, 1], np.trunc(exact).astype(np.int32))
+    # roundf ties away from zero
+    np.testing.assert_array_equal(oracle._round_half_away(np.array([2.5, -2.5, 0.5, -0.49])),
+                                  [3.0, -3.0, 1.0, -0.0])

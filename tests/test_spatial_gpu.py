@@ -222,15 +222,17 @@ def test_diff_batch_is_arg_error(af, rng):
     assert e.value.code == af.AF_ERR_ARG
 
 
-def test_auto_routes_to_freq_not_supported(af, rng):
+def test_auto_routes_to_freq(af, rng):
+    # a non-square filter wider than 5 goes to the frequency domain under
+    # AUTO (src/api/c/convolve.cpp:99-130): same result as the spatial call
     x = rng.random((40, 40)).astype(np.float32)
-    w = rng.random((7, 3)).astype(np.float32)  # non-square > 5 -> FREQ under AUTO
+    w = rng.random((7, 3)).astype(np.float32)
     assert oracle.is_freq_domain(2, (40, 40, 1, 1), (7, 3, 1, 1), 0)
-    with pytest.raises(af.AfError) as e:
-        af.convolve2(x, w, 0, 0)
-    assert e.value.code == af.AF_ERR_NOT_SUPPORTED
-    out = af.convolve2(x, w, 0, SPATIAL).to_numpy()
-    assert oracle.rel_l2(out, oracle.convolve(x, w, 2, False)) <= 1e-5
+    auto = af.convolve2(x, w, 0, 0).to_numpy()
+    spatial = af.convolve2(x, w, 0, SPATIAL).to_numpy()
+    assert auto.shape == spatial.shape
+    assert oracle.rel_l2(auto, spatial) <= 1e-5
+    assert oracle.rel_l2(auto, oracle.convolve(x, w, 2, False)) <= 1e-5
 
 
 def test_f16_signal_is_type_error(af, rng):
