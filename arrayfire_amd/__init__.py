@@ -60,12 +60,15 @@ EXPORTED = [
     # include/af/device.h
     "af_init", "af_get_device_count", "af_set_device", "af_get_device", "af_sync",
     "af_device_array", "af_get_device_ptr", "af_alloc_host", "af_free_host",
-    "af_device_gc", "af_device_mem_info",
+    "af_device_gc", "af_device_mem_info", "af_lock_array", "af_unlock_array",
+    "af_is_locked_array",
     # include/af/signal.h
     "af_convolve1", "af_convolve2", "af_convolve3", "af_convolve2_sep",
-    "af_fft_convolve1", "af_fft_convolve2", "af_fft_convolve3",
+    "af_fft_convolve1", "af_fft_convolve2", "af_fft_convolve3", "af_fir", "af_iir",
     # include/af/ml.h
     "af_convolve2_nn", "af_convolve2_gradient_nn",
+    # include/af/image.h
+    "af_unwrap", "af_wrap", "af_wrap_v2", "af_gaussian_kernel", "af_dog",
     # include/af/hip.h
     "afhip_get_stream", "af_cast", "afhip_convolve2_nn_ex",
     "afhip_convolve2_gradient_nn_ex",
@@ -80,6 +83,8 @@ EXPORTED = [
     "afhip_k_convolve2_ragged", "afhip_k_conv2_nn_patch_pack", "afhip_k_conv2_nn_patch_run",
     "afhip_k_conv2_nn_wgrad_rows_partial", "afhip_k_conv2_nn_wgrad_rows_reduce",
     "afhip_k_fft_convolve_workspace", "afhip_k_fft_convolve",
+    "afhip_k_unwrap", "afhip_k_wrap", "afhip_k_region_copy", "afhip_k_sub", "afhip_k_div_cols",
+    "afhip_k_iir_workspace", "afhip_k_iir",
 ]
 
 
@@ -121,11 +126,23 @@ def lib() -> ctypes.CDLL:
         L.af_get_device_ptr.argtypes = [ctypes.POINTER(vp), vp]
         L.af_device_array.argtypes = [ctypes.POINTER(vp), vp, u, i64p, ctypes.c_int]
         L.af_cast.argtypes = [ctypes.POINTER(vp), vp, ctypes.c_int]
+        L.af_lock_array.argtypes = [vp]
+        L.af_unlock_array.argtypes = [vp]
+        L.af_is_locked_array.argtypes = [ctypes.POINTER(ctypes.c_bool), vp]
         for n in ("af_convolve1", "af_convolve2", "af_convolve3"):
             getattr(L, n).argtypes = [ctypes.POINTER(vp), vp, vp, ctypes.c_int, ctypes.c_int]
         for n in ("af_fft_convolve1", "af_fft_convolve2", "af_fft_convolve3"):
             getattr(L, n).argtypes = [ctypes.POINTER(vp), vp, vp, ctypes.c_int]
         L.af_convolve2_sep.argtypes = [ctypes.POINTER(vp), vp, vp, vp, ctypes.c_int]
+        ll6 = [ctypes.c_longlong] * 6
+        L.af_unwrap.argtypes = [ctypes.POINTER(vp), vp] + ll6 + [ctypes.c_bool]
+        L.af_wrap.argtypes = [ctypes.POINTER(vp), vp] + [ctypes.c_longlong] * 8 + [ctypes.c_bool]
+        L.af_wrap_v2.argtypes = [ctypes.POINTER(vp), vp] + [ctypes.c_longlong] * 8 + [ctypes.c_bool]
+        L.af_fir.argtypes = [ctypes.POINTER(vp), vp, vp]
+        L.af_iir.argtypes = [ctypes.POINTER(vp), vp, vp, vp]
+        L.af_dog.argtypes = [ctypes.POINTER(vp), vp, ctypes.c_int, ctypes.c_int]
+        L.af_gaussian_kernel.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_double, ctypes.c_double]
         L.af_convolve2_nn.argtypes = [ctypes.POINTER(vp), vp, vp, u, i64p, u, i64p, u, i64p]
         L.afhip_convolve2_nn_ex.argtypes = [ctypes.POINTER(vp), vp, vp, u, i64p, u, i64p, u,
                                             i64p, ctypes.c_int, ctypes.c_int]
@@ -371,6 +388,51 @@ def convolve2GradientNN(incoming_gradient, original_signal, original_filter, con
     return _out(lib().afhip_convolve2_gradient_nn_ex, g.handle, s.handle, f.handle, o.handle,
                 len(stride), st, len(padding), pd, len(dilation), dl, grad_type,
                 precision or 0, -1 if out_type is None else out_type)
+
+
+def unwrap(a, wx, wy, sx, sy, px=0, py=0, is_column=True) -> Array:
+    """af::unwrap (src/api/cpp/unwrap.cpp): patches as columns (or rows)."""
+    x = _arr(a)
+    return _out(lib().af_unwrap, x.handle, wx, wy, sx, sy, px, py, bool(is_column))
+
+
+def wrap(a, ox, oy, wx, wy, sx, sy, px=0, py=0, is_column=True, out: Array | None = None) -> Array:
+    """af::wrap; with ``out`` the patches are added into that array in place
+    (af_wrap_v2 with an existing output) and it is returned."""
+    x = _arr(a)
+    if out is None:
+        return _out(lib().af_wrap, x.handle, ox, oy, wx, wy, sx, sy, px, py, bool(is_column))
+    h = ctypes.c_void_p(out.handle.value)
+    check(lib().af_wrap_v2(ctypes.byref(h), x.handle, ox, oy, wx, wy, sx, sy, px, py,
+                           bool(is_column)))
+    return out
+
+
+def fir(b, x) -> Array:
+    """af::fir (src/api/cpp/signal.cpp): y = the first x0 rows of
+    convolve1(x, b, EXPAND)."""
+    bb, xx = _arr(b), _arr(x)
+    return _out(lib().af_fir, bb.handle, xx.handle)
+
+
+def iir(b, a, x) -> Array:
+    """af::iir: feedforward b, feedback a (a[0] normalises)."""
+    bb, aa, xx = _arr(b), _arr(a), _arr(x)
+    return _out(lib().af_iir, bb.handle, aa.handle, xx.handle)
+
+
+def dog(image, radius1: int, radius2: int) -> Array:
+    """af::dog: difference of Gaussian smoothings."""
+    x = _arr(image)
+    return _out(lib().af_dog, x.handle, int(radius1), int(radius2))
+
+
+def gaussian_kernel(rows: int, cols: int, sigma_r: float = 0.0, sigma_c: float = 0.0) -> Array:
+    """af::gaussianKernel."""
+    h = ctypes.c_void_p()
+    check(lib().af_gaussian_kernel(ctypes.byref(h), int(rows), int(cols), float(sigma_r),
+                                   float(sigma_c)))
+    return Array(handle=h.value)
 
 
 class RaggedPlan:

@@ -192,7 +192,7 @@ void pool_info(size_t *alloc_bytes, size_t *alloc_buffers, size_t *lock_bytes,
 }
 
 Buffer::~Buffer() {
-    if (!ptr) return;
+    if (!ptr || locked) return;  // a locked buffer belongs to the caller
     if (user_owned) {
         if (g_stream) (void)hipStreamSynchronize(g_stream);
         (void)hipFree(ptr);
@@ -434,6 +434,36 @@ AFAPI af_err af_device_array(af_array *arr, void *data, const unsigned ndims,
         p->buf->bytes = (size_t)p->dims.elements() * dtype_size(type);
         p->buf->user_owned = true;
         *arr = register_impl(p);
+    }
+    CATCHALL;
+    return AF_SUCCESS;
+}
+
+// include/af/device.h:543-565: a locked buffer is left to the caller (never
+// freed or returned to the pool); unlocking hands it back to the library.
+AFAPI af_err af_lock_array(const af_array arr) {
+    try {
+        ArrayImpl &a = get(arr);
+        if (a.buf) a.buf->locked = true;
+    }
+    CATCHALL;
+    return AF_SUCCESS;
+}
+
+AFAPI af_err af_unlock_array(const af_array arr) {
+    try {
+        ArrayImpl &a = get(arr);
+        if (a.buf) a.buf->locked = false;
+    }
+    CATCHALL;
+    return AF_SUCCESS;
+}
+
+AFAPI af_err af_is_locked_array(bool *res, const af_array arr) {
+    try {
+        ARG_ASSERT(0, res != nullptr);
+        ArrayImpl &a = get(arr);
+        *res = a.buf && a.buf->locked;
     }
     CATCHALL;
     return AF_SUCCESS;

@@ -108,6 +108,7 @@ struct Buffer {
     void *ptr = nullptr;
     size_t bytes = 0;
     bool user_owned = false;  // af_device_array: hipFree on release
+    bool locked = false;      // af_lock_array: never freed or pooled by the library
     ~Buffer();
 };
 

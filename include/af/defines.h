@@ -13,6 +13,9 @@
 #pragma once
 
 #include <stddef.h>
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
 
 #ifdef __cplusplus
 #define AFAPI extern "C" __attribute__((visibility("default")))

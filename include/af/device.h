@@ -18,6 +18,13 @@ AFAPI af_err af_sync(const int device);
  * ownership and frees it with hipFree when the last handle is released */
 AFAPI af_err af_device_array(af_array *arr, void *data, const unsigned ndims,
                              const dim_t *const dims, const af_dtype type);
+/* include/af/device.h:543 -- the array's buffer is no longer freed by the
+ * library (also not when the last handle is released) */
+AFAPI af_err af_lock_array(const af_array arr);
+/* include/af/device.h:553 -- hands the buffer back to the library */
+AFAPI af_err af_unlock_array(const af_array arr);
+/* include/af/device.h:565 */
+AFAPI af_err af_is_locked_array(bool *res, const af_array arr);
 /* include/af/device.h:576 -- raw device pointer of an array */
 AFAPI af_err af_get_device_ptr(void **ptr, const af_array arr);
 /* include/af/device.h (af_alloc_host / af_free_host) -- host memory for

@@ -245,3 +245,55 @@ AFHIP_KAPI int afhip_k_conv2_nn_wgrad_rows_reduce(
     const long long *wdims, const long long *ydims, const long long *stride,
     const long long *pad, const long long *dil, const void *workspace,
     long long workspace_bytes, void *stream);
+
+/* unwrap<T> (src/backend/cpu/unwrap.cpp:20-40, kernel/unwrap.hpp:20-80):
+ * im2col with strides, zero padding and dilation (dx, dy); odims must be
+ * [wx*wy, nx*ny, C, N] (is_column) or [nx*ny, wx*wy, C, N]. Any dtype (the
+ * elements are moved as bytes). */
+AFHIP_KAPI int afhip_k_unwrap(int dtype, void *out, const long long *odims,
+                              const void *in, const long long *idims,
+                              long long wx, long long wy, long long sx,
+                              long long sy, long long px, long long py,
+                              long long dx, long long dy, int is_column,
+                              void *stream);
+
+/* wrap<T> (src/backend/cpu/wrap.cpp:20-40, kernel/wrap.hpp:20-80): col2im.
+ * out [ox, oy, C, N] receives (accumulate ? out : 0) plus every patch
+ * element that lands on it, added in the reference's patch order in the
+ * element type. Every dtype but f16 / bf16. */
+AFHIP_KAPI int afhip_k_wrap(int dtype, void *out, const long long *odims,
+                            const void *in, const long long *idims,
+                            long long wx, long long wy, long long sx,
+                            long long sy, long long px, long long py,
+                            int is_column, int accumulate, void *stream);
+
+/* af_index of a region (src/api/c/index.cpp), as af_fir uses it: copies the
+ * odims-sized block at `offsets` of a contiguous idims array into a
+ * contiguous out. esize is the element size in bytes (1, 2, 4, 8 or 16). */
+AFHIP_KAPI int afhip_k_region_copy(int esize, void *out, const long long *odims,
+                                   const void *in, const long long *idims,
+                                   const long long *offsets, void *stream);
+
+/* out = a - b over n elements of f32 or f64 (arithOp<af_sub_t>, as af_dog
+ * uses it, src/api/c/dog.cpp:50). */
+AFHIP_KAPI int afhip_k_sub(int dtype, void *out, const void *a, const void *b,
+                           long long n, void *stream);
+
+/* out = b / a with a broadcast along b's columns: a holds one element per
+ * column (per_col) or one in total (af_div(b, a, true) in af_iir,
+ * src/api/c/iir.cpp:76-81). f32, f64, c32, c64. */
+AFHIP_KAPI int afhip_k_div_cols(int dtype, void *out, const void *b,
+                                const long long *bdims, const void *a,
+                                int per_col, void *stream);
+
+/* kernel::iir<T> (src/backend/cpu/kernel/iir.hpp:20-55): the feedback
+ * recurrence over the first ydims[0] rows of each column of c (the EXPAND
+ * convolution of x with b, dims cdims); a is [na] (one for all columns) or
+ * [na, ydims[1], ydims[2], ydims[3]]. f32, f64, c32, c64. More than 16
+ * feedback taps need a workspace of afhip_k_iir_workspace bytes. */
+AFHIP_KAPI long long afhip_k_iir_workspace(int dtype, long long na,
+                                           const long long *ydims);
+AFHIP_KAPI int afhip_k_iir(int dtype, void *y, const long long *ydims,
+                           const void *c, const long long *cdims, const void *a,
+                           const long long *adims, void *workspace,
+                           long long workspace_bytes, void *stream);

@@ -381,6 +381,91 @@ def wrap_dilated(cols, ox, oy, wx, wy, sx, sy, px, py, dx, dy) -> np.ndarray:
     return out.reshape(od, order="F")
 
 
+def _patch_index(sd, wx, wy, sx, sy, px, py):
+    """For every (col, loc) of af_unwrap's patch grid: the flat (x, y) index
+    into one [d0, d1] plane and whether it lies inside the image
+    (src/backend/cpu/kernel/unwrap.hpp:40-72 with dilation 1)."""
+    nx = nn_out_size(sd[0], wx, sx, px, 1)
+    ny = nn_out_size(sd[1], wy, sy, py, 1)
+    col = np.arange(nx * ny)
+    winy, winx = col // nx, col % nx
+    loc = np.arange(wx * wy)
+    y, x = loc // wx, loc % wx
+    xpad = (winx * sx - px)[:, None] + x[None, :]
+    ypad = (winy * sy - py)[:, None] + y[None, :]
+    inside = (xpad >= 0) & (xpad < sd[0]) & (ypad >= 0) & (ypad < sd[1])
+    flat = np.where(inside, ypad * sd[0] + xpad, 0)
+    return nx, ny, flat, inside        # [col, loc]
+
+
+def unwrap_any(x, wx, wy, sx, sy, px, py, is_column=True) -> np.ndarray:
+    """af_unwrap for any dtype (src/api/c/unwrap.cpp:37-99): a gather, so
+    exact. Output [wx*wy, nx*ny, C, N] (is_column) or [nx*ny, wx*wy, C, N]."""
+    x = as4(x)
+    sd = dims4(x)
+    nx, ny, flat, inside = _patch_index(sd, wx, wy, sx, sy, px, py)
+    planes = x.reshape(sd[0] * sd[1], sd[2], sd[3], order="F")
+    g = planes[flat.ravel()].reshape(flat.shape + (sd[2], sd[3]))     # [col, loc, C, N]
+    g = np.where(inside[:, :, None, None], g, np.zeros((), dtype=x.dtype))
+    return np.ascontiguousarray(g.transpose(1, 0, 2, 3) if is_column else g)
+
+
+def wrap_any(cols, ox, oy, wx, wy, sx, sy, px, py, is_column=True, out=None) -> np.ndarray:
+    """af_wrap for any dtype (src/api/c/wrap.cpp:26-115, kernel/wrap.hpp:20-80):
+    patch elements added into `out` (zeros by default) in the reference's
+    serial order. Floating types run the C restatement (the order fixes the
+    rounding; complex parts add independently); integer types add modulo
+    2^bits, where order does not matter."""
+    cols = as4(cols)
+    if is_column:
+        cols = np.ascontiguousarray(cols.transpose(1, 0, 2, 3))     # -> [nx*ny, wx*wy, C, N]
+    idm = dims4(cols)
+    od = (ox, oy, idm[2], idm[3])
+    dt = cols.dtype
+    base = np.zeros(od, dtype=dt) if out is None else as4(out).astype(dt, copy=True)
+    if dt in (np.float32, np.float64):
+        if out is None:
+            return wrap_dilated(cols, ox, oy, wx, wy, sx, sy, px, py, 1, 1).astype(dt)
+        return _wrap_serial(cols, base, wx, wy, sx, sy, px, py)   # the serial sum from `out`
+    if dt in (np.complex64, np.complex128):
+        re = wrap_any(np.ascontiguousarray(cols.real), ox, oy, wx, wy, sx, sy, px, py, False,
+                      None if out is None else base.real.copy())
+        im = wrap_any(np.ascontiguousarray(cols.imag), ox, oy, wx, wy, sx, sy, px, py, False,
+                      None if out is None else base.imag.copy())
+        return (re + 1j * im).astype(dt)
+    u = np.dtype(f"u{dt.itemsize}") if dt != np.bool_ else np.dtype(np.uint8)
+    nx, ny, flat, inside = _patch_index((ox, oy), wx, wy, sx, sy, px, py)
+    acc = base.view(np.int8).view(u) if dt == np.bool_ else base.view(u)
+    acc = acc.reshape(ox * oy, idm[2], idm[3], order="F").copy()
+    src = cols.view(np.int8).view(u) if dt == np.bool_ else cols.view(u)
+    for c in range(idm[2]):
+        for n in range(idm[3]):
+            v = src[:, :, c, n][inside]
+            np.add.at(acc[:, c, n], flat[inside], v)
+    res = acc.reshape(od, order="F")
+    return res.view(np.int8) if dt == np.bool_ else res.view(dt)
+
+
+def _wrap_serial(cols, base, wx, wy, sx, sy, px, py):
+    """The serial scatter-add of kernel/wrap.hpp:40-72 from a given start
+    (pure Python over patches; small cases only)."""
+    idm = dims4(cols)
+    ox, oy = base.shape[0], base.shape[1]
+    out = base.copy()
+    nx = nn_out_size(ox, wx, sx, px, 1)
+    for n in range(idm[3]):
+        for c in range(idm[2]):
+            for col in range(idm[0]):
+                winy, winx = divmod(col, nx)
+                for y in range(wy):
+                    yp = winy * sy - py + y
+                    for x in range(wx):
+                        xp = winx * sx - px + x
+                        if 0 <= xp < ox and 0 <= yp < oy:
+                            out[xp, yp, c, n] = out[xp, yp, c, n] + cols[col, y * wx + x, c, n]
+    return out
+
+
 def _fmod(a, shape):
     """modDims: reinterpret in column-major order."""
     return np.reshape(a, shape, order="F")
@@ -550,3 +635,57 @@ def gaussian_kernel(rows: int, cols: int, sigma_r: float = 0.0,
         t = w * w + t
     t = np.exp(-(dtype(0.5) * t)).astype(dtype)
     return (t / t.sum(dtype=dtype)).astype(dtype)
+
+
+# ------------------------------------------------ internal callers
+def fir(b, x) -> np.ndarray:
+    """af_fir (src/api/c/iir.cpp:24-41): the first x0 rows of the EXPAND
+    convolution of x with b (spatial restatement; the library may route long
+    filters or many filters through the frequency domain, as AUTO does)."""
+    x4, b4 = as4(x), as4(b)
+    r = convolve_dispatch(x4, b4, 1, True)
+    return np.ascontiguousarray(r[:dims4(x4)[0]])
+
+
+def iir(b, a, x) -> np.ndarray:
+    """af_iir (src/api/c/iir.cpp:49-103, src/backend/cpu/iir.cpp:23-45,
+    kernel/iir.hpp:20-55): c = first x0 rows of conv(x, b, EXPAND) with the
+    batch kind of iir.cpp:25-28, then per column
+        y[i] = (c[i] + z[0]) / a[0];  z[k-1] = z[k] - a[k] * y[i]  (k >= 1)
+    evaluated in the element type. Pure Python over rows: small cases."""
+    x4, b4, a4 = as4(x), as4(b), as4(a)
+    if dims4(a4)[0] == 1:
+        bn = b4 / (a4 if a4.size == 1 else a4.reshape((1,) + a4.shape[1:]))
+        return fir(bn.astype(b4.dtype), x4)
+    # iir.cpp:25-28's batch kind (NONE / SAME / RHS / LHS by ndims) equals
+    # identify_batch_kind's for every shape af_iir accepts
+    c = convolve(x4, b4, 1, True)
+    x0 = dims4(x4)[0]
+    c = c[:x0]
+    y = np.empty_like(c)
+    na = dims4(a4)[0]
+    per_col = a4.size > na
+    for l in range(c.shape[3]):
+        for k in range(c.shape[2]):
+            for j in range(c.shape[1]):
+                av = a4[:, j, k, l] if per_col else a4[:, 0, 0, 0]
+                z = np.zeros(na, dtype=c.dtype)
+                col = c[:, j, k, l]
+                out = y[:, j, k, l]
+                for i in range(x0):
+                    v = (col[i] + z[0]) / av[0]
+                    out[i] = v
+                    for q in range(1, na):
+                        z[q - 1] = z[q] - av[q] * v
+    return y
+
+
+def dog(x, r1: int, r2: int) -> np.ndarray:
+    """af_dog (src/api/c/dog.cpp:30-80): accT smoothings with the (2r+1)^2
+    Gaussian windows (sigma 0.25 * side) and their difference."""
+    x4 = as4(x)
+    at = np.float64 if x4.dtype == np.float64 else np.float32
+    xa = x4.astype(at)
+    g1 = gaussian_kernel(2 * r1 + 1, 2 * r1 + 1).astype(at)
+    g2 = gaussian_kernel(2 * r2 + 1, 2 * r2 + 1).astype(at)
+    return convolve(xa, g1, 2, False) - convolve(xa, g2, 2, False)

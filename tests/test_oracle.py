@@ -373,3 +373,143 @@ def test_fft_convolve_output_rules():
     # roundf ties away from zero
     np.testing.assert_array_equal(oracle._round_half_away(np.array([2.5, -2.5, 0.5, -0.49])),
                                   [3.0, -3.0, 1.0, -0.0])
+
+
+# ---------------------------------------------------------------- unwrap / wrap
+def _wrap_factor(ix, iy, wx, wy, sx, sy, px, py):
+    """How many patches cover each pixel: the count test/wrap.cpp:88-108
+    builds for its wrap(unwrap(x)) == x * factor check."""
+    f = np.zeros((ix, iy), dtype=np.int64)
+    nx = (ix + 2 * px - wx) // sx + 1
+    ny = (iy + 2 * py - wy) // sy + 1
+    for wyi in range(ny):
+        for wxi in range(nx):
+            x0, y0 = wxi * sx - px, wyi * sy - py
+            xs = [x for x in range(x0, x0 + wx) if 0 <= x < ix]
+            ys = [y for y in range(y0, y0 + wy) if 0 <= y < iy]
+            for y in ys:
+                f[xs, y] += 1
+    return f
+
+
+def test_unwrap_wrap_doc_snippet():
+    """The known answers of test/wrap.cpp:200-246 (af::wrap documentation)."""
+    a = np.arange(1, 10, dtype=np.float32).reshape(3, 3, order="F")
+    u = oracle.unwrap_any(a, 2, 2, 2, 2, 1, 1, True)[:, :, 0, 0]
+    np.testing.assert_array_equal(u, [[0, 0, 0, 5], [0, 0, 4, 6], [0, 2, 0, 8], [1, 3, 7, 9]])
+    np.testing.assert_array_equal(oracle.wrap_any(u, 3, 3, 2, 2, 2, 2, 1, 1, True)[:, :, 0, 0], a)
+    b = np.ones((3, 3), np.float32)
+    ub = oracle.unwrap_any(b, 2, 2, 1, 1, 0, 0, True)
+    np.testing.assert_array_equal(ub[:, :, 0, 0], np.ones((4, 4)))
+    np.testing.assert_array_equal(oracle.wrap_any(ub, 3, 3, 2, 2, 1, 1, 0, 0, True)[:, :, 0, 0],
+                                  [[1, 2, 1], [2, 4, 2], [1, 2, 1]])
+
+
+# (ix, iy, wx, wy, sx, sy, px, py): a spread of test/wrap.cpp:159-181's WRAP_INIT cases
+WRAP_GEOMS = [(30, 10, 3, 3, 1, 1, 0, 0), (30, 10, 3, 3, 2, 2, 1, 1), (30, 10, 4, 4, 2, 2, 3, 3),
+              (30, 10, 8, 8, 8, 8, 7, 7), (33, 21, 8, 12, 8, 12, 7, 11), (30, 20, 15, 10, 15, 10, 14, 9)]
+
+
+@pytest.mark.parametrize("g", WRAP_GEOMS)
+@pytest.mark.parametrize("is_column", [True, False])
+def test_wrap_of_unwrap_is_factor_times_input(g, is_column):
+    ix, iy, wx, wy, sx, sy, px, py = g
+    rng = np.random.default_rng(7)
+    x = rng.integers(-50, 50, (ix, iy, 2)).astype(np.int32)
+    u = oracle.unwrap_any(x, wx, wy, sx, sy, px, py, is_column)
+    r = oracle.wrap_any(u, ix, iy, wx, wy, sx, sy, px, py, is_column)
+    f = _wrap_factor(ix, iy, wx, wy, sx, sy, px, py)
+    np.testing.assert_array_equal(r[:, :, :, 0], x * f[:, :, None])
+    rf = oracle.wrap_any(u.astype(np.float32), ix, iy, wx, wy, sx, sy, px, py, is_column)
+    np.testing.assert_array_equal(rf[:, :, :, 0], (x * f[:, :, None]).astype(np.float32))
+
+
+def test_unwrap_any_matches_c_restatement():
+    rng = np.random.default_rng(8)
+    x = rng.random((17, 13, 3), dtype=np.float32)
+    for g in [(3, 3, 1, 1, 1, 1), (5, 4, 2, 3, 2, 1), (4, 4, 4, 4, 0, 0)]:
+        a = oracle.unwrap_any(x, *g, is_column=False)
+        b = oracle.unwrap(x, *g, 1, 1)
+        np.testing.assert_array_equal(a, b)
+
+
+def test_wrap_serial_from_zero_matches_c_restatement():
+    rng = np.random.default_rng(9)
+    cols = rng.standard_normal((20, 9, 2, 1)).astype(np.float32)   # 5x4 patches of 3x3 on 11x9
+    a = oracle.wrap_any(cols, 11, 9, 3, 3, 2, 2, 0, 0, False)
+    b = oracle._wrap_serial(cols, np.zeros((11, 9, 2, 1), np.float32), 3, 3, 2, 2, 0, 0)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_unwrap_wrap_adjoint():
+    """<unwrap(x), c> == <x, wrap(c)> in FP64."""
+    rng = np.random.default_rng(10)
+    x = rng.standard_normal((23, 17, 2))
+    for g in [(3, 3, 1, 1, 1, 1), (5, 3, 2, 3, 4, 2), (6, 6, 6, 6, 0, 0)]:
+        u = oracle.unwrap_any(x, *g, is_column=True)
+        c = rng.standard_normal(u.shape)
+        w = oracle.wrap_any(c, 23, 17, *g, is_column=True)
+        assert abs(np.vdot(u, c) - np.vdot(oracle.as4(x), w)) <= 1e-9 * np.abs(u).sum() * 10
+
+
+def test_wrap_integer_types_wrap_around():
+    """Integer wrap adds modulo 2^bits, as C++ += on the reference's types."""
+    u = np.full((4, 4, 1, 1), 200, dtype=np.uint8)      # 2x2 windows, stride 1 on 3x3
+    r = oracle.wrap_any(u, 3, 3, 2, 2, 1, 1, 0, 0, True)
+    np.testing.assert_array_equal(r[:, :, 0, 0], np.array([[200, 144, 200], [144, 32, 144],
+                                                           [200, 144, 200]], dtype=np.uint8))
+
+
+# ---------------------------------------------------------------- FIR / IIR / DoG
+def test_iir_oracle_is_lfilter():
+    """The reference's recurrence (kernel/iir.hpp) is the direct-form
+    difference equation a0 y[n] = sum b x - sum_{k>=1} a_k y[n-k]: scipy's
+    lfilter computes the same filter."""
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal((200, 3))
+    b = rng.standard_normal(5)
+    a = np.array([1.7, -0.4, 0.2, 0.05])
+    y = oracle.iir(b, a, x)[:, :, 0, 0]
+    ref = np.stack([ss.lfilter(b, a, x[:, j]) for j in range(3)], axis=1)
+    np.testing.assert_allclose(y, ref, rtol=1e-10, atol=1e-12)
+    # per-column feedforward and feedback taps (the SAME batch)
+    B = rng.standard_normal((4, 3))
+    A = np.stack([np.array([2.0, 0.3, -0.1]), np.array([1.0, -0.5, 0.25]),
+                  np.array([-1.5, 0.2, 0.1])], axis=1)
+    y = oracle.iir(B, A, x)[:, :, 0, 0]
+    ref = np.stack([ss.lfilter(B[:, j], A[:, j], x[:, j]) for j in range(3)], axis=1)
+    np.testing.assert_allclose(y, ref, rtol=1e-10, atol=1e-12)
+
+
+def test_iir_single_tap_is_normalised_fir():
+    rng = np.random.default_rng(12)
+    x = rng.standard_normal(64)
+    b = rng.standard_normal(7)
+    y = oracle.iir(b, np.array([2.5]), x)[:, 0, 0, 0]
+    np.testing.assert_allclose(y, np.convolve(x, b / 2.5)[:64], rtol=1e-12, atol=1e-12)
+
+
+def test_fir_is_truncated_full_convolution():
+    rng = np.random.default_rng(13)
+    x = rng.standard_normal((50, 2))
+    b = rng.standard_normal((6, 2))
+    y = oracle.fir(b, x)[:, :, 0, 0]
+    ref = np.stack([np.convolve(x[:, j], b[:, j])[:50] for j in range(2)], axis=1)
+    np.testing.assert_allclose(y, ref, rtol=1e-12, atol=1e-12)
+
+
+def test_dog_constant_image_and_scipy():
+    """DoG of a constant image is zero away from the borders (both windows
+    sum to 1), and equals the difference of two zero-padded scipy
+    convolutions everywhere."""
+    x = np.ones((40, 30), np.float32)
+    d = oracle.dog(x, 3, 2)[:, :, 0, 0]
+    assert np.abs(d[7:-7, 7:-7]).max() < 1e-6
+    rng = np.random.default_rng(14)
+    x = rng.random((40, 30))
+    d = oracle.dog(x, 2, 1)[:, :, 0, 0]
+    # the windows are af_gaussian_kernel's f32 values, cast to the f64 input type
+    g1 = oracle.gaussian_kernel(5, 5).astype(np.float64)
+    g2 = oracle.gaussian_kernel(3, 3).astype(np.float64)
+    ref = ss.convolve2d(x, g1, mode="same") - ss.convolve2d(x, g2, mode="same")
+    np.testing.assert_allclose(d, ref, rtol=1e-12, atol=1e-12)
