@@ -50,20 +50,42 @@ __global__ __launch_bounds__(256) void sub_kernel(T *__restrict__ out, const T *
 
 // complex arithmetic of the recurrence (std::complex semantics for + - *;
 // the division is the textbook (c * conj(d)) / |d|^2 form)
+// (each body switches contraction off: a pragma in the caller does not reach
+// into these functions, and a product fused into the next subtraction would
+// round differently from the reference)
 template <typename T>
 struct IirOps {
-    __device__ static T add(T a, T b) { return a + b; }
-    __device__ static T sub(T a, T b) { return a - b; }
-    __device__ static T mul(T a, T b) { return a * b; }
-    __device__ static T div(T a, T b) { return a / b; }
+    __device__ static T add(T a, T b) {
+#pragma clang fp contract(off)
+        return a + b;
+    }
+    __device__ static T sub(T a, T b) {
+#pragma clang fp contract(off)
+        return a - b;
+    }
+    __device__ static T mul(T a, T b) {
+#pragma clang fp contract(off)
+        return a * b;
+    }
+    __device__ static T div(T a, T b) {
+#pragma clang fp contract(off)
+        return a / b;
+    }
 };
 template <typename R>
 struct IirOps<cplx<R>> {
     typedef cplx<R> C;
-    __device__ static C add(C a, C b) { return C(a.re + b.re, a.im + b.im); }
-    __device__ static C sub(C a, C b) { return C(a.re - b.re, a.im - b.im); }
+    __device__ static C add(C a, C b) {
+#pragma clang fp contract(off)
+        return C(a.re + b.re, a.im + b.im);
+    }
+    __device__ static C sub(C a, C b) {
+#pragma clang fp contract(off)
+        return C(a.re - b.re, a.im - b.im);
+    }
     __device__ static C mul(C a, C b) { return a * b; }
     __device__ static C div(C a, C b) {
+#pragma clang fp contract(off)
         const R den = b.re * b.re + b.im * b.im;
         return C((a.re * b.re + a.im * b.im) / den, (a.im * b.re - a.re * b.im) / den);
     }

@@ -680,12 +680,16 @@ def iir(b, a, x) -> np.ndarray:
     return y
 
 
-def dog(x, r1: int, r2: int) -> np.ndarray:
+def dog(x, r1: int, r2: int, windows=None) -> np.ndarray:
     """af_dog (src/api/c/dog.cpp:30-80): accT smoothings with the (2r+1)^2
-    Gaussian windows (sigma 0.25 * side) and their difference."""
+    f32 Gaussian windows (sigma 0.25 * side) and their difference.
+    ``windows`` replaces the two f32 windows (to take the library's own
+    af_gaussian_kernel values, which may differ from gaussian_kernel()'s in
+    the last bit of an exp)."""
     x4 = as4(x)
     at = np.float64 if x4.dtype == np.float64 else np.float32
     xa = x4.astype(at)
-    g1 = gaussian_kernel(2 * r1 + 1, 2 * r1 + 1).astype(at)
-    g2 = gaussian_kernel(2 * r2 + 1, 2 * r2 + 1).astype(at)
+    if windows is None:
+        windows = (gaussian_kernel(2 * r1 + 1, 2 * r1 + 1), gaussian_kernel(2 * r2 + 1, 2 * r2 + 1))
+    g1, g2 = (np.asarray(w, dtype=np.float32).astype(at) for w in windows)
     return convolve(xa, g1, 2, False) - convolve(xa, g2, 2, False)

@@ -89,7 +89,9 @@ def test_iir_recurrence(af, t, na):
     y = af.iir(b, a, x).to_numpy()
     ref = oracle.iir(b, a, x)
     assert y.shape == ref.shape == (400, 3, 1, 1)
-    assert oracle.rel_l2(y, ref) <= bound * 10
+    e = oracle.rel_l2(y, ref)
+    print(f"iir {t} na={na}: rel-L2 {e:.3e}, max |diff| {np.abs(y - ref).max():.3e}", flush=True)
+    assert e <= bound * 10
 
 
 @pytest.mark.parametrize("t", ["f32", "f64"])
@@ -124,7 +126,11 @@ def test_dog(af, dt, shape):
     for r1, r2 in [(3, 2), (1, 5)]:
         d = af.dog(x, r1, r2)
         assert d.type == (af.f64 if dt == np.float64 else af.f32)
-        ref = oracle.dog(x, r1, r2)
+        # the oracle smooths with the library's own f32 windows (pinned to
+        # the closed form in test_gaussian_kernel): a last-bit difference in
+        # a window is amplified by the difference of two close smoothings
+        win = tuple(af.gaussian_kernel(2 * r + 1, 2 * r + 1).to_numpy()[:, :, 0, 0] for r in (r1, r2))
+        ref = oracle.dog(x, r1, r2, windows=win)
         got = d.to_numpy()
         assert got.shape == ref.shape
         bound = 1e-12 if dt == np.float64 else 1e-5
