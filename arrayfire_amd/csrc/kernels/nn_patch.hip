@@ -15,6 +15,7 @@
 // buffer (one 16-byte load per lane, prefetched a step ahead, double-buffered
 // in LDS); B fragments are read straight from the patch at the tap's offset.
 #include <cstdlib>
+#include <mutex>
 #include <type_traits>
 #include <utility>
 
@@ -1205,4 +1206,51 @@ extern "C" int afhip_k_conv2_nn_patch_run(int pass, int in_type, int out_type, i
                                           long long workspace_bytes, void *stream) {
     return patch_call(2, pass, in_type, out_type, precision, out, in, nullptr, xd, wd, yd, st, pd, dl,
                       const_cast<void *>(workspace), workspace_bytes, stream);
+}
+
+// ---------------------------------------------------------------- no-alloc variant
+// afhip_k_conv2_nn_patch_noalloc: the same launches with the packed-weight
+// workspace in a file-scope device array (one copy per GPU, held until the
+// process ends) instead of a caller or library allocation, for callers that
+// cannot allocate (code under HIP graph capture, tight serving loops). Its
+// device address is looked up once per device with that device current, and
+// only that address is passed to kernels. Calls that overlap on one GPU
+// (different streams, or host threads) share the array and race; calls
+// queued in order on one stream do not. Nothing in it needs to start at
+// zero: the pack kernel writes every byte the convolution reads.
+namespace afhip {
+constexpr long long kNoAllocScratchBytes = 16LL << 20;
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[64] = {};
+
+static int noalloc_scratch(void **ptr) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return K_ERR_RUNTIME;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr) return K_ERR_RUNTIME;
+        g_scratch_ptr[dev] = p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    return K_OK;
+}
+}  // namespace afhip
+
+extern "C" long long afhip_k_conv2_nn_patch_noalloc_limit(void) { return kNoAllocScratchBytes; }
+
+extern "C" int afhip_k_conv2_nn_patch_noalloc(int pass, int in_type, int out_type, int precision, void *out,
+                                              const void *in, const void *w, const long long *xd,
+                                              const long long *wd, const long long *yd, const long long *st,
+                                              const long long *pd, const long long *dl, void *stream) {
+    const long long need = afhip_k_conv2_nn_patch_workspace(pass, in_type, precision, xd, wd, yd, st, pd, dl);
+    if (need == 0) return K_ERR_NOT_SUPPORTED;
+    if (need > kNoAllocScratchBytes) return K_ERR_SIZE;
+    void *ws = nullptr;
+    const int rc = noalloc_scratch(&ws);
+    if (rc != K_OK) return rc;
+    return patch_call(3, pass, in_type, out_type, precision, out, in, w, xd, wd, yd, st, pd, dl, ws,
+                      kNoAllocScratchBytes, stream);
 }

@@ -173,6 +173,24 @@ AFHIP_KAPI int afhip_k_conv2_nn_patch(int pass, int in_type, int out_type,
                                       const long long *dil, void *workspace,
                                       long long workspace_bytes, void *stream);
 
+/* No counterpart in the reference: afhip_k_conv2_nn_patch without the
+ * workspace arguments and without any allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). The packed weights go to a per-GPU static device
+ * array of afhip_k_conv2_nn_patch_noalloc_limit() bytes (16 MiB); a layer
+ * whose workspace query exceeds it returns 203. Calls that overlap on one
+ * GPU share that array and race; calls in order on one stream do not. */
+AFHIP_KAPI long long afhip_k_conv2_nn_patch_noalloc_limit(void);
+AFHIP_KAPI int afhip_k_conv2_nn_patch_noalloc(int pass, int in_type, int out_type,
+                                              int precision, void *out,
+                                              const void *in, const void *w,
+                                              const long long *xdims,
+                                              const long long *wdims,
+                                              const long long *ydims,
+                                              const long long *stride,
+                                              const long long *pad,
+                                              const long long *dil, void *stream);
+
 /* Row-staged filter gradient for 16-bit operands (bfloat16 or binary16
  * storage, FP32 accumulation): the fast path behind afhip_k_conv2_nn_wgrad
  * for filters of width 1 or 3, dilation 1 along W, stride 1 or 2 along W and
