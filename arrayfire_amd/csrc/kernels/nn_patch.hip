@@ -907,6 +907,329 @@ __global__ __launch_bounds__(Q_THREADS, 2) void conv_pipe(TOut *__restrict__ y,
     }
 }
 
+// ---------------------------------------------------------------- weights-resident kernel
+// conv_wres: the forward pass for GEMMs whose reduction (NCH channel chunks
+// of 32 x 9 taps) fits in registers, 16-bit operands.
+// Unit of work: (tile of 32*NQ filters, image, band of RB output rows); one
+// persistent workgroup of 8 waves per CU walks units lb, lb + G, ... in
+// filter-tile order. Wave w: wq = w % NQ picks 32 filters of the tile, pg =
+// w / NQ one of PG = 8 / NQ pixel groups. Each wave keeps the B fragments
+// of its 32 filters for every k-step in VGPRs (NCH*9 x 2 fragments of 4
+// VGPRs), reloaded only when the unit's filter tile changes, so the MFMA
+// loop reads nothing but the patch from LDS: per k-step 2 A fragments (16
+// pixels x 32 channels at the tap's offset, read a k-step ahead) feed 4
+// MFMAs, with no barrier. The band's patch (every channel chunk; channel-last
+// 64-byte pixel rows swizzled as q_swz, the segment transposes of conv_pipe)
+// arrives in two row pieces: the loads of both are issued at once, the first
+// piece is stored, and the output rows it covers are computed while the
+// second piece's loads land; then the second piece is stored and the
+// remaining rows are computed. Wave w walks the 16-pixel blocks pg, pg + PG,
+// ... in pairs, then singly, and stores each finished pair straight from the
+// accumulators (4 consecutive output pixels of one filter per lane).
+constexpr int WR_THREADS = 512;
+constexpr int WR_WAVES = WR_THREADS / 64;
+
+struct WresParams {
+    int W, H, C, N, OW, OH, K;
+    int s0, s1, p0, p1, d1;
+    int PW, PR, RB;           // patch pitch (pixels, multiple of 8), patch rows, band rows
+    int PRa, RBa;             // rows of the first patch piece, output rows it covers
+    int nseg, nbands, ntw;    // 16-pixel input blocks per row, bands per image, filter tiles
+    long long nunits;         // ntw * N * nbands
+    int w_bytes;              // packed weight bytes (buffer resource size)
+    int vec;                  // OW % 4 == 0 and one output image under 2^31 - 64 bytes: buffer stores
+};
+
+template <int MODE, typename TIn, typename TOut, int NCH, int NQ, int PASS>
+__global__ __launch_bounds__(WR_THREADS, 1) void conv_wres(TOut *__restrict__ y, const TIn *__restrict__ x,
+                                                           const char *__restrict__ wpk, WresParams p) {
+    static_assert(MODE == PM_BF16 || MODE == PM_F16, "16-bit operands");
+    constexpr int RS = 9, NKS = NCH * RS, SEG = 8, KC = 32, NF = 2, PG = WR_WAVES / NQ;
+    constexpr int kOOB = 0x7fffffc0;
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // uniform: slot arithmetic stays scalar
+    const int wq = wave % NQ, pg = wave / NQ;
+    const int g = lane >> 4;
+    const int chunk_bytes = p.PR * p.PW * Q_PB;
+    const long long HW = (long long)p.H * p.W;
+    const int P = p.OH * p.OW;
+    const int G = gridDim.x;
+    const int lb = (int)xcd_remap(blockIdx.x, G);
+
+    // zero the patch once: columns outside the gathered blocks stay zero
+    for (int i = tid; i < NCH * chunk_bytes / 16; i += WR_THREADS)
+        reinterpret_cast<uint4 *>(smem)[i] = make_uint4(0, 0, 0, 0);
+    __syncthreads();   // before any wave stores gathered data over the zeros
+
+    pu16x8 wb[NKS][NF];
+    int cur_kt = -1;
+    const auto wr = __builtin_amdgcn_make_buffer_rsrc(const_cast<char *>(wpk), (short)0, p.w_bytes, 0x00020000);
+    auto load_weights = [&](int kt) __attribute__((always_inline)) {
+        // packed [kt128][chunk][tap][128 rows][32 ch]; this wave's filters
+        // f0 + fi * 16 + lane % 16 with f0 = kt * 32 * NQ + wq * 32 stay in one
+        // 128-row panel
+        const int f0 = kt * 32 * NQ + wq * 32;
+        const int k128 = f0 >> 7, row = (f0 & 127) + (lane & 15);
+#pragma unroll
+        for (int ks = 0; ks < NKS; ++ks)
+#pragma unroll
+            for (int fi = 0; fi < NF; ++fi) {
+                const int panel = (k128 * NCH + ks / RS) * RS + ks % RS;
+                const int off = panel * (P_BM * KC * 2) + (row + fi * 16) * (KC * 2) + g * 16;
+                wb[ks][fi] = __builtin_bit_cast(pu16x8, __builtin_amdgcn_raw_buffer_load_b128(wr, off, 0, 0));
+            }
+    };
+
+    const int ch_l = g * SEG + (lane & (SEG - 1));
+    const int px_l = (lane & 15) & ~(SEG - 1);
+    const int x_img_bytes = p.C * (int)HW * 2;
+    const int rowb = p.d1 * p.PW * Q_PB;   // bytes per filter-row step in the patch
+
+    for (long long u = lb; u < p.nunits; u += G) {
+        const int band = (int)(u % p.nbands);
+        const long long r1 = u / p.nbands;
+        const int n = (int)(r1 % p.N), kt = (int)(r1 / p.N);
+        const int r0 = band * p.RB;
+        const int rows = min(p.RB, p.OH - r0);
+        const int Pu = rows * p.OW;            // output pixels of the unit
+        const int ih0 = r0 * p.s1 - p.p1;
+        // ---- gather: slot (chunk, patch row, 16-pixel block) of a piece of
+        // patch rows [lo, lo + nr); WS slots per wave per round, slot s of a
+        // round at rest = s0 + s * 8 + wave (decoded incrementally: uniform)
+        const auto xr = __builtin_amdgcn_make_buffer_rsrc(const_cast<TIn *>(x + (long long)n * p.C * HW), (short)0,
+                                                          x_img_bytes, 0x00020000);
+        struct Slot {
+            int cc, row, blk;
+        };
+        auto decode = [&](auto ws, Slot (&sl)[decltype(ws)::value], int s0, int lo, int nr)
+                          __attribute__((always_inline)) {
+            constexpr int WS = decltype(ws)::value;
+            const int per_chunk = nr * p.nseg, rest = s0 + wave;
+            int cc = rest / per_chunk, rem = rest - cc * per_chunk;
+            int row = rem / p.nseg, blk = rem - row * p.nseg;
+#pragma unroll
+            for (int s = 0; s < WS; ++s) {
+                sl[s] = Slot{cc, lo + row, blk};
+                blk += WR_WAVES;
+                while (blk >= p.nseg) {
+                    blk -= p.nseg;
+                    ++row;
+                }
+                while (row >= nr) {
+                    row -= nr;
+                    ++cc;
+                }
+            }
+        };
+        auto issue = [&](auto ws, uint4 (&gx)[decltype(ws)::value], const Slot (&sl)[decltype(ws)::value])
+                         __attribute__((always_inline)) {
+            constexpr int WS = decltype(ws)::value;
+#pragma unroll
+            for (int s = 0; s < WS; ++s) {
+                const int c = sl[s].cc * KC + ch_l;
+                const int ih = ih0 + sl[s].row, iw = sl[s].blk * 16 + px_l;
+                const bool ok = sl[s].cc < NCH && ih >= 0 && ih < p.H && c < p.C && iw < p.W;
+                const int off = ok ? (c * (int)HW + ih * p.W + iw) * 2 : kOOB;
+                gx[s] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(xr, off, 0, 0));
+            }
+        };
+        auto store_slots = [&](auto ws, uint4 (&gx)[decltype(ws)::value], const Slot (&sl)[decltype(ws)::value])
+                               __attribute__((always_inline)) {
+            constexpr int WS = decltype(ws)::value;
+#pragma unroll
+            for (int s = 0; s < WS; ++s) {
+                if (sl[s].cc < NCH) {  // wave-uniform: the transpose sees every lane
+                    uint4 v = gx[s];
+                    q_tr8x8_b16(v, lane);
+                    const int col = p.p0 + sl[s].blk * 16 + (lane & 15);
+                    if (col >= 0 && col < p.PW)
+                        *reinterpret_cast<uint4 *>(smem + sl[s].cc * chunk_bytes + q_swz(sl[s].row * p.PW + col, g)) =
+                            v;
+                }
+            }
+        };
+        using W8 = std::integral_constant<int, 8>;
+        using W4 = std::integral_constant<int, 4>;
+        const int PRa = min(p.PRa, p.PR), PRb = p.PR - PRa;
+        const int slots_a = NCH * PRa * p.nseg, slots_b = NCH * PRb * p.nseg;
+        const bool overlap_b = slots_b > 0 && slots_b <= 4 * WR_WAVES;   // one round of 4 slots
+        // loads in the order they are needed: the first piece, the weights
+        // (when the filter tile changes; the first MFMA waits for them), then
+        // the second piece
+        uint4 gb[4];
+        {
+            uint4 ga[8];
+            Slot sa[8];
+            decode(W8{}, sa, 0, 0, PRa);
+            issue(W8{}, ga, sa);
+            if (kt != cur_kt) {                // uniform across the workgroup
+                load_weights(kt);
+                cur_kt = kt;
+            }
+            if (overlap_b) {
+                Slot sb[4];
+                decode(W4{}, sb, 0, PRa, PRb);
+                issue(W4{}, gb, sb);
+            }
+            store_slots(W8{}, ga, sa);
+            for (int s0 = 8 * WR_WAVES; s0 < slots_a; s0 += 8 * WR_WAVES) {
+                decode(W8{}, sa, s0, 0, PRa);
+                issue(W8{}, ga, sa);
+                store_slots(W8{}, ga, sa);
+            }
+        }
+        __syncthreads();
+
+        // ---- compute: 16-pixel blocks pg, pg + PG, ... of the unit; blocks
+        // below nb1 read only the first piece
+        const int nb = (Pu + 15) >> 4;
+        const int nb1 = min(nb, (min(p.RBa, rows) * p.OW) >> 4);
+        const int mine = nb > pg ? (nb - pg + PG - 1) / PG : 0;
+        const int mine1 = nb1 > pg ? (nb1 - pg + PG - 1) / PG : 0;
+        // the lane's pixel q = b * 16 + lane % 16 of its current block as (oh,
+        // ow), walked PG * 16 pixels per block without a division
+        int oh, ow;
+        {
+            const int q0 = pg * 16 + (lane & 15);
+            oh = q0 / p.OW;
+            ow = q0 - oh * p.OW;
+        }
+        auto next_block = [&](int (&pb)[3]) __attribute__((always_inline)) {
+            int o = oh, w = ow;
+            if (o >= rows) {            // dead pixels read the unit's last live one; never stored
+                o = rows - 1;
+                w = p.OW - 1;
+            }
+            const int base = o * p.s1 * p.PW + w * p.s0;
+#pragma unroll
+            for (int ph = 0; ph < 3; ++ph) pb[ph] = q_swz(base + ph, g);
+            ow += PG * 16;
+            while (ow >= p.OW) {
+                ow -= p.OW;
+                ++oh;
+            }
+        };
+        // output offsets (elements, within image n) of this lane's filters;
+        // -1 for filters past K
+        int kfo[NF];
+#pragma unroll
+        for (int fi = 0; fi < NF; ++fi) {
+            const int kf = kt * 32 * NQ + wq * 32 + fi * 16 + (lane & 15);
+            kfo[fi] = kf < p.K ? kf * P + r0 * p.OW : -1;
+        }
+        TOut *yn = y + (long long)n * p.K * P;
+        const auto yr = __builtin_amdgcn_make_buffer_rsrc(yn, (short)0, p.vec ? p.K * P * (int)sizeof(TOut) : 0,
+                                                          0x00020000);
+        auto store = [&](int b, int fi, const pf32x4 &v) __attribute__((always_inline)) {
+            const int q = b * 16 + 4 * (lane >> 4);
+            if (p.vec) {
+                // OW % 4 == 0: the 4 pixels are all live or all dead, and
+                // 16-byte (8-byte) aligned; dead lanes store past the end
+                const int off = (kfo[fi] >= 0 && q < Pu) ? (kfo[fi] + q) * (int)sizeof(TOut) : kOOB;
+                if constexpr (sizeof(TOut) == 4) {
+                    __builtin_amdgcn_raw_buffer_store_b128(
+                        __builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned int, v), yr, off, 0, 0);
+                } else {
+                    __attribute__((ext_vector_type(2))) unsigned int pk;
+                    if constexpr (MODE == PM_F16) {
+                        pk.x = (uint32_t)pf32_to_f16_bits(v[0]) | ((uint32_t)pf32_to_f16_bits(v[1]) << 16);
+                        pk.y = (uint32_t)pf32_to_f16_bits(v[2]) | ((uint32_t)pf32_to_f16_bits(v[3]) << 16);
+                    } else {
+                        pk.x = (uint32_t)f32_to_bf16_bits(v[0]) | ((uint32_t)f32_to_bf16_bits(v[1]) << 16);
+                        pk.y = (uint32_t)f32_to_bf16_bits(v[2]) | ((uint32_t)f32_to_bf16_bits(v[3]) << 16);
+                    }
+                    __builtin_amdgcn_raw_buffer_store_b64(pk, yr, off, 0, 0);
+                }
+                return;
+            }
+            if (kfo[fi] < 0 || q >= Pu) return;
+            TOut *d = yn + kfo[fi] + q;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (q + k < Pu) {
+                    if constexpr (sizeof(TOut) == 4)
+                        d[k] = v[k];
+                    else
+                        reinterpret_cast<uint16_t *>(d)[k] =
+                            MODE == PM_F16 ? pf32_to_f16_bits(v[k]) : f32_to_bf16_bits(v[k]);
+                }
+            }
+        };
+        auto mfma = [&](const pu16x8 &a, const pu16x8 &b, pf32x4 &acc) __attribute__((always_inline)) {
+            if constexpr (MODE == PM_BF16)
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(pbf16x8, a),
+                                                              __builtin_bit_cast(pbf16x8, b), acc, 0, 0, 0);
+            else
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(pf16x8, a),
+                                                             __builtin_bit_cast(pf16x8, b), acc, 0, 0, 0);
+        };
+        // NB blocks at once: k-steps (chunk, filter row, column phase) fully
+        // unrolled so every weight fragment is a fixed register; the A
+        // fragments of k-step ks + 1 are read while the MFMAs of ks run
+        auto run = [&](auto nbk, int i0) __attribute__((always_inline)) {
+            constexpr int NB = decltype(nbk)::value;
+            int pa[NB][3];
+#pragma unroll
+            for (int bi = 0; bi < NB; ++bi) next_block(pa[bi]);
+            pf32x4 acc[NB][NF];
+#pragma unroll
+            for (int bi = 0; bi < NB; ++bi)
+#pragma unroll
+                for (int fi = 0; fi < NF; ++fi) acc[bi][fi] = pf32x4{0.f, 0.f, 0.f, 0.f};
+            constexpr int PD = 2;   // k-steps of LDS reads in flight ahead of the MFMAs
+            pu16x8 a[PD + 1][NB];
+            auto fetch = [&](int ks, pu16x8 (&dst)[NB]) __attribute__((always_inline)) {
+                const int cc = ks / RS, t = ks % RS;
+                const char *base = smem + cc * chunk_bytes + (t / 3) * rowb;
+#pragma unroll
+                for (int bi = 0; bi < NB; ++bi) dst[bi] = *reinterpret_cast<const pu16x8 *>(base + pa[bi][t % 3]);
+            };
+#pragma unroll
+            for (int ks = 0; ks < PD; ++ks) fetch(ks, a[ks]);
+#pragma unroll
+            for (int ks = 0; ks < NKS; ++ks) {
+                // the LDS reads of k-step ks + PD are issued before this
+                // k-step's MFMAs (the barrier keeps the scheduler from sinking
+                // them), so each read has PD k-steps of MFMAs to land
+                if (ks + PD < NKS) fetch(ks + PD, a[(ks + PD) % (PD + 1)]);
+                __builtin_amdgcn_sched_barrier(0x0006);   // only VALU / SALU work may cross
+#pragma unroll
+                for (int bi = 0; bi < NB; ++bi)
+#pragma unroll
+                    for (int fi = 0; fi < NF; ++fi) mfma(a[ks % (PD + 1)][bi], wb[ks][fi], acc[bi][fi]);
+            }
+#pragma unroll
+            for (int bi = 0; bi < NB; ++bi)
+#pragma unroll
+                for (int fi = 0; fi < NF; ++fi) store(pg + PG * (i0 + bi), fi, acc[bi][fi]);
+        };
+        int i0 = 0;
+        for (; i0 + 2 <= mine1; i0 += 2) run(std::integral_constant<int, 2>{}, i0);
+        for (; i0 < mine1; ++i0) run(std::integral_constant<int, 1>{}, i0);
+        if (PRb > 0) {
+            // the second piece: its loads have been landing during the first
+            // piece's MFMAs (or are issued now when they take several rounds)
+            if (overlap_b) {
+                Slot sb[4];
+                decode(W4{}, sb, 0, PRa, PRb);
+                store_slots(W4{}, gb, sb);
+            } else {
+                for (int s0 = 0; s0 < slots_b; s0 += 4 * WR_WAVES) {
+                    Slot sb[4];
+                    decode(W4{}, sb, s0, PRa, PRb);
+                    issue(W4{}, gb, sb);
+                    store_slots(W4{}, gb, sb);
+                }
+            }
+            __syncthreads();
+        }
+        for (; i0 + 2 <= mine; i0 += 2) run(std::integral_constant<int, 2>{}, i0);
+        for (; i0 < mine; ++i0) run(std::integral_constant<int, 1>{}, i0);
+        __syncthreads();   // the next unit's gather overwrites the patch
+    }
+}
+
 template <int MODE>
 static size_t lds_bytes(const PatchParams &p) {
     using Cfg = PCfg<MODE>;
@@ -1041,6 +1364,93 @@ static int launch_pipe(void *y, const void *x, const void *ws, const PatchParams
     return launch_status();
 }
 
+// Geometry of the weights-resident forward kernel. Served: 16-bit operands
+// stored as such, W a whole number of 8-pixel segments, a 16-byte aligned
+// input, 3x3 filters with dilation 1 along W, C <= 64 (at most 2 chunks of 32
+// channels: 18 k-steps of weights in registers), and a band of output rows
+// whose patch (every chunk) fits in LDS. Filter tiles are 128 filters wide
+// (4 x 32) when K > 64, else 64 (2 x 32, with 4 pixel groups). The band
+// height RB minimises the units per CU x (blocks per unit + a per-unit
+// overhead of about two blocks); the first patch piece covers the band's
+// first ceil(RB / 2) output rows. AFHIP_NN_WRES=0 in the environment
+// disables the kernel.
+template <int MODE, typename TIn>
+static bool plan_wres(WresParams &w, int &nch, int &nq, const PatchParams &p, const void *x, int dgrad,
+                      long long wsz) {
+    if constexpr (MODE == PM_SPLIT || MODE == PM_F32 || sizeof(TIn) != 2) {
+        return false;
+    } else {
+        const char *off = std::getenv("AFHIP_NN_WRES");
+        if (off && off[0] == '0') return false;
+        if (dgrad) return false;
+        if (p.fw != 3 || p.fh != 3 || p.d0 != 1 || p.C > 64 || p.C <= 0) return false;
+        if (p.W % 8 != 0 || (reinterpret_cast<uintptr_t>(x) & 15) != 0) return false;
+        if (p.p0 < 0 || p.p1 < 0) return false;
+        nch = (p.C + 31) / 32;
+        nq = p.K > 64 ? 4 : 2;
+        w.W = p.W; w.H = p.H; w.C = p.C; w.N = p.N; w.OW = p.OW; w.OH = p.OH; w.K = p.K;
+        w.s0 = p.s0; w.s1 = p.s1; w.p0 = p.p0; w.p1 = p.p1; w.d1 = p.d1;
+        w.nseg = (p.W + 15) / 16;
+        // pitch: every image column plus the widest tap reach, rounded up to a
+        // multiple of 8 pixels so that a filter-row step keeps the swizzle;
+        // gathered columns past the pitch (the dead end of the last 16-pixel
+        // block) are not stored
+        const int need = max((p.OW - 1) * p.s0 + 2 * p.d0 + 1, p.p0 + p.W);
+        w.PW = (need + 7) & ~7;
+        w.ntw = (p.K + 32 * nq - 1) / (32 * nq);
+        const long long cus = device_cu_count();
+        long long best = -1;
+        for (int rb = 1; rb <= p.OH; ++rb) {
+            const int pr = (rb - 1) * p.s1 + 2 * p.d1 + 1;
+            if ((size_t)nch * pr * w.PW * Q_PB > kPatchLdsLimit) break;
+            const long long nb = (p.OH + rb - 1) / rb;
+            const long long units = (long long)w.ntw * p.N * nb;
+            const long long per_cu = (units + cus - 1) / cus;
+            const long long cost = per_cu * (((long long)rb * p.OW + 15) / 16 + 2);
+            if (best < 0 || cost <= best) {
+                best = cost;
+                w.RB = rb;
+            }
+        }
+        if (best < 0) return false;
+        w.PR = (w.RB - 1) * p.s1 + 2 * p.d1 + 1;
+        w.RBa = (w.RB + 1) / 2;
+        w.PRa = (w.RBa - 1) * p.s1 + 2 * p.d1 + 1;
+        if (w.PRa >= w.PR) w.RBa = w.RB, w.PRa = w.PR;
+        w.nbands = (p.OH + w.RB - 1) / w.RB;
+        w.nunits = (long long)w.ntw * p.N * w.nbands;
+        if (wsz >= (1LL << 31) - 64 || (long long)p.C * p.H * p.W * 2 >= (1LL << 31) - 64) return false;
+        if ((long long)p.N * p.K * p.OH * p.OW >= (1LL << 40)) return false;
+        w.w_bytes = (int)wsz;
+        w.vec = 0;   // set by the launcher from the output
+        return true;
+    }
+}
+
+template <int MODE, typename TIn, typename TOut>
+static int launch_wres(void *y, const void *x, const void *ws, const WresParams &w, int nch, int nq,
+                       hipStream_t s) {
+    using Fn = void (*)(TOut *, const TIn *, const char *, WresParams);
+    const Fn fn = nch == 1 ? (nq == 4 ? conv_wres<MODE, TIn, TOut, 1, 4, 0> : conv_wres<MODE, TIn, TOut, 1, 2, 0>)
+                           : (nq == 4 ? conv_wres<MODE, TIn, TOut, 2, 4, 0> : conv_wres<MODE, TIn, TOut, 2, 2, 0>);
+    const size_t lds = (size_t)nch * w.PR * w.PW * Q_PB;
+    static bool attr_set[4] = {false, false, false, false};
+    const int slot = (nch - 1) * 2 + (nq == 4);
+    if (!attr_set[slot]) {
+        if (hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)kPatchLdsLimit) != hipSuccess)
+            return K_ERR_RUNTIME;
+        attr_set[slot] = true;
+    }
+    long long grid = device_cu_count();
+    if (grid > w.nunits) grid = w.nunits;
+    WresParams q = w;
+    q.vec = (w.OW % 4 == 0) && (reinterpret_cast<uintptr_t>(y) & 15) == 0 &&
+            (long long)w.K * w.OH * w.OW * (long long)sizeof(TOut) < (1LL << 31) - 64;
+    fn<<<dim3((unsigned)grid), WR_THREADS, lds, s>>>((TOut *)y, (const TIn *)x, (const char *)ws, q);
+    return launch_status();
+}
+
 // what: 1 = pack the weights into ws, 2 = run the convolution with the
 // weights already packed in ws, 3 = both.
 template <int MODE, typename TIn, typename TOut>
@@ -1061,6 +1471,13 @@ static int launch(void *y, const void *x, const void *w, PatchParams &p, int dgr
         PatchParams q;
         int wm = 0;
         if constexpr (MODE != PM_SPLIT && sizeof(TIn) == PCfg<MODE>::ESZ) {
+            if constexpr (MODE == PM_BF16 || MODE == PM_F16) {
+                WresParams wr;
+                int nch = 0, nq = 0;
+                const long long wsz = (long long)p.ktiles * p.nchunks * p.RS * P_BM * PCfg<MODE>::KC * 2;
+                if (plan_wres<MODE, TIn>(wr, nch, nq, p, x, dgrad, wsz))
+                    return launch_wres<MODE, TIn, TOut>(y, x, ws, wr, nch, nq, s);
+            }
             if (plan_pipe<MODE, TIn>(q, p, x, wm)) return launch_pipe<MODE, TIn, TOut>(y, x, ws, q, wm, dgrad, s);
         }
         const size_t lds = lds_bytes<MODE>(p);
