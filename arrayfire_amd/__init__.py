@@ -85,7 +85,8 @@ EXPORTED = [
     "afhip_k_fft_convolve_workspace", "afhip_k_fft_convolve",
     "afhip_k_unwrap", "afhip_k_wrap", "afhip_k_region_copy", "afhip_k_sub", "afhip_k_div_cols",
     "afhip_k_iir_workspace", "afhip_k_iir", "afhip_k_conv2_nn_patch_noalloc_limit",
-    "afhip_k_conv2_nn_patch_noalloc",
+    "afhip_k_conv2_nn_patch_noalloc", "afhip_k_conv2_nn_dgrad_poly_workspace",
+    "afhip_k_conv2_nn_dgrad_poly",
 ]
 
 
@@ -172,6 +173,9 @@ def lib() -> ctypes.CDLL:
         L.afhip_k_conv2_nn_wgrad_rows_workspace.argtypes = [ctypes.c_int] * 2 + [i64p] * 6
         L.afhip_k_conv2_nn_wgrad_rows_workspace.restype = ll
         L.afhip_k_conv2_nn_wgrad_rows.argtypes = [ctypes.c_int] * 3 + [vp, vp, vp] + [i64p] * 6 + [vp, ll, vp]
+        L.afhip_k_conv2_nn_dgrad_poly_workspace.argtypes = [ctypes.c_int] * 3 + [i64p] * 6
+        L.afhip_k_conv2_nn_dgrad_poly_workspace.restype = ll
+        L.afhip_k_conv2_nn_dgrad_poly.argtypes = [ctypes.c_int] * 3 + [vp, vp, vp] + [i64p] * 6 + [vp, ll, vp]
         L.afhip_k_conv2_nn_patch_noalloc.argtypes = [ctypes.c_int] * 4 + [vp, vp, vp] + [i64p] * 6 + [vp]
         L.afhip_k_conv2_nn_patch_noalloc_limit.argtypes = []
         L.afhip_k_conv2_nn_patch_noalloc_limit.restype = ll

@@ -254,6 +254,20 @@ static af_array conv2_data_grad(const ArrayImpl &dy, const ArrayImpl &x, const A
         return out.release();
     }
     if (run_patch(1, dy, w, out_type, precision, o, g)) return out.release();
+    // strided layers: the polyphase split into stride-1 convolutions on the
+    // patch kernels (AFHIP_NN_GATHER=1 keeps the gather kernel)
+    const char *gather = std::getenv("AFHIP_NN_GATHER");
+    if (!(gather && gather[0] == '1')) {
+        const long long pb = afhip_k_conv2_nn_dgrad_poly_workspace(x.type, out_type, precision, g.x.d, g.w.d,
+                                                                  g.y.d, g.st, g.pd, g.dl);
+        if (pb > 0) {
+            auto ws = scratch((size_t)pb);
+            check_k(afhip_k_conv2_nn_dgrad_poly(x.type, out_type, precision, o, dy.data(), w.data(), g.x.d, g.w.d,
+                                                g.y.d, g.st, g.pd, g.dl, ws->ptr, pb, stream()),
+                    "convolve2_gradient_nn");
+            return out.release();
+        }
+    }
     check_k(afhip_k_conv2_nn_dgrad(x.type, out_type, precision, o, dy.data(), w.data(), g.x.d, g.w.d,
                                    g.y.d, g.st, g.pd, g.dl, stream()),
             "convolve2_gradient_nn");

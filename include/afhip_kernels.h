@@ -315,3 +315,28 @@ AFHIP_KAPI int afhip_k_iir(int dtype, void *y, const long long *ydims,
                            const void *c, const long long *cdims, const void *a,
                            const long long *adims, void *workspace,
                            long long workspace_bytes, void *stream);
+
+/* conv2DataGradient<T> for stride > 1 and dilation 1
+ * (src/backend/cpu/convolve.cpp:179-211) by polyphase decomposition: one
+ * stride-1 convolution of dy per residue pair (u mod s0, v mod s1) with the
+ * filter taps that reach those positions (channel dims transposed), on the
+ * patch-staged forward kernels, each scattered into the interleaved dx. The
+ * workspace holds one phase's sub-filter, packed weights and output; the
+ * query returns 0 when the geometry is not served (stride 1, dilation > 1,
+ * a phase the patch kernels do not take), and the call then returns 301. */
+AFHIP_KAPI long long afhip_k_conv2_nn_dgrad_poly_workspace(
+    int in_type, int out_type, int precision, const long long *xdims,
+    const long long *wdims, const long long *ydims, const long long *stride,
+    const long long *pad, const long long *dil);
+AFHIP_KAPI int afhip_k_conv2_nn_dgrad_poly(int in_type, int out_type,
+                                           int precision, void *dx,
+                                           const void *dy, const void *w,
+                                           const long long *xdims,
+                                           const long long *wdims,
+                                           const long long *ydims,
+                                           const long long *stride,
+                                           const long long *pad,
+                                           const long long *dil,
+                                           void *workspace,
+                                           long long workspace_bytes,
+                                           void *stream);
