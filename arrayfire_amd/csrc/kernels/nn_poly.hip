@@ -17,6 +17,7 @@
 // output into the interleaved dx. Every dx element belongs to exactly one
 // phase and is that phase's complete sum, rounded once, as in the reference.
 #include <algorithm>
+#include <cstdlib>
 
 #include "kcommon.hpp"
 #include "../../../include/afhip_kernels.h"
@@ -112,11 +113,30 @@ struct PolyWs {
     long long g, pack, yp, total;
 };
 
+// The phases run one after the other, each on workgroups of 128 output
+// channels x 256 pixels of one image: a layer whose phases give fewer of
+// those than there are CUs leaves the chip mostly idle and is served faster
+// by the gather kernel, which the caller then uses.
+static int cu_count() {
+    static int cus[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (cus[dev] <= 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cus[dev] = n;
+    }
+    return cus[dev];
+}
+
 static bool poly_plan(int in_type, int out_type, int precision, const long long *xd, const long long *wd,
                       const long long *yd, const long long *st, const long long *pd, const long long *dl,
                       PolyWs &ws) {
     if (dl[0] != 1 || dl[1] != 1 || (st[0] == 1 && st[1] == 1) || st[0] < 1 || st[1] < 1) return false;
     if (pd[0] < 0 || pd[1] < 0) return false;
+    const long long tiles = ((xd[2] + 127) / 128) * ((yd[0] * yd[1] + 255) / 256) * yd[3];
+    const char *force = std::getenv("AFHIP_NN_POLY");
+    if (!(force && force[0] == '1') && tiles < cu_count()) return false;
     const int ei = esize(in_type), eo = esize(out_type);
     if (!ei || !eo) return false;
     ws = PolyWs{0, 0, 0, 0};

@@ -190,25 +190,75 @@ def time_step(af, ev, stream, step, steps, warmup):
     return total, per_kernel, wall
 
 
-def time_af_api(af, ev, stream, X, W, DY, out_type, steps, warmup=2):
+def af_step(af, X, W, DY, out_type, precision=None, geom=(ST, PD, DL)):
+    """One step through the af_* C API: (y, dx, dw)."""
+    st, pd, dl = geom
+    y = af.convolve2NN(X, W, st, pd, dl, precision=precision, out_type=out_type)
+    dx = af.convolve2GradientNN(DY, X, W, y, st, pd, dl, af.AF_CONV_GRADIENT_DATA, precision=precision,
+                                out_type=out_type)
+    dw = af.convolve2GradientNN(DY, X, W, y, st, pd, dl, af.AF_CONV_GRADIENT_FILTER, precision=precision,
+                                out_type=out_type)
+    return [y, dx, dw]
+
+
+def time_af_api(af, ev, stream, X, W, DY, out_type, steps, warmup=2, precision=None):
     """The step through the af_* C API (pool allocation and dispatch on every
     call), timed with one HIP event pair around all steps."""
     hold = []
     for _ in range(warmup):
-        y = af.convolve2NN(X, W, ST, PD, DL, out_type=out_type)
-        hold = [y, af.convolve2GradientNN(DY, X, W, y, ST, PD, DL, af.AF_CONV_GRADIENT_DATA, out_type=out_type),
-                af.convolve2GradientNN(DY, X, W, y, ST, PD, DL, af.AF_CONV_GRADIENT_FILTER, out_type=out_type)]
+        hold = af_step(af, X, W, DY, out_type, precision)
     af.sync()
     e0, e1 = ev.create(2)
     ev.record(e0, stream)
     for _ in range(steps):
-        y = af.convolve2NN(X, W, ST, PD, DL, out_type=out_type)
-        hold = [y, af.convolve2GradientNN(DY, X, W, y, ST, PD, DL, af.AF_CONV_GRADIENT_DATA, out_type=out_type),
-                af.convolve2GradientNN(DY, X, W, y, ST, PD, DL, af.AF_CONV_GRADIENT_FILTER, out_type=out_type)]
+        hold = af_step(af, X, W, DY, out_type, precision)
     ev.record(e1, stream)
     ms = ev.ms(e0, e1) / steps
     del hold
     return ms
+
+
+def time_cold(af, ev, stream, X, W, DY, out_type, reps=5):
+    """The step with the 256 MB MALL and the L2s flushed before it: a 1 GiB
+    fill runs first (outside the timed region), then one step between an
+    event pair. Median of reps."""
+    flush = af.constant(0, (256 * 1024 * 1024,), af.f32)
+    L = af.lib()
+    hold = af_step(af, X, W, DY, out_type)
+    af.sync()
+    ts = []
+    for i in range(reps):
+        assert L.afhip_k_fill(af.f32, ctypes.c_void_p(flush.device_ptr()), float(i), 256 * 1024 * 1024,
+                              ctypes.c_void_p(stream)) == 0
+        e0, e1 = ev.create(2)
+        ev.record(e0, stream)
+        hold = af_step(af, X, W, DY, out_type)
+        ev.record(e1, stream)
+        ts.append(ev.ms(e0, e1))
+    del hold, flush
+    return float(np.median(ts))
+
+
+def host_dispatch_us(af):
+    """Host time to issue one call (no synchronisation): af_convolve2 on a
+    64x64 image with a 5x5 filter and af_convolve2_nn of the headline layer,
+    first call in the process and the mean of 200 (resp. 20) later calls."""
+    import time as _t
+    x = af.Array(np.ones((64, 64), np.float32))
+    w = af.Array(np.ones((5, 5), np.float32))
+    af.sync()
+    t0 = _t.perf_counter()
+    o = af.convolve2(x, w, af.AF_CONV_DEFAULT, af.AF_CONV_SPATIAL)
+    first = (_t.perf_counter() - t0) * 1e6
+    af.sync()
+    hold = []
+    t0 = _t.perf_counter()
+    for _ in range(200):
+        hold.append(af.convolve2(x, w, af.AF_CONV_DEFAULT, af.AF_CONV_SPATIAL))
+    per = (_t.perf_counter() - t0) / 200 * 1e6
+    af.sync()
+    del hold, o
+    return {"convolve2_64sq_5x5_first_call": round(first, 1), "convolve2_64sq_5x5": round(per, 2)}
 
 
 def fp64_errors(x, w, dy, outs):
@@ -278,12 +328,7 @@ def cpu_baseline(budget_s=15.0):
     SGEMM + col2im) on a bounded sample: the same layer with fewer images."""
     import oracle
 
-    try:
-        from threadpoolctl import threadpool_info
-        threads = max([i.get("num_threads", 1) for i in threadpool_info()
-                       if i.get("user_api") == "blas"] or [1])
-    except Exception:  # pragma: no cover
-        threads = os.cpu_count() or 1
+    threads = _blas_threads()
     rng = np.random.default_rng(SEED)
     n = 4
     xd = XD[:3] + (n,)
@@ -307,11 +352,41 @@ def cpu_baseline(budget_s=15.0):
                       "im2col/col2im single-threaded C, SGEMM on numpy's OpenBLAS"}
 
 
-def spatial_secondary(af, ev, stream, reps=10):
+def _cpu_threads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def cpu_rate(fn, items, units_per_item, budget_s=1.5, threads=1):
+    """Runs fn over items (a thread pool of `threads` workers; the oracle's C
+    loops release the GIL) until budget_s has passed; returns units / s."""
+    import concurrent.futures as cf
+
+    done, t = 0, 0.0
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(threads) as ex:
+        while t < budget_s:
+            list(ex.map(fn, items))
+            done += len(items)
+            t = time.perf_counter() - t0
+    return units_per_item * done / t
+
+
+def cpu_line(fn, items, units_per_item, unit, sample, budget_s=1.5):
+    """The reference-faithful single-thread rate (the CPU backend runs each
+    kernel on one worker thread, src/backend/cpu/queue.hpp:98-114) and the
+    same loops over a thread pool on the host's cores."""
+    one = cpu_rate(fn, items[:1], units_per_item, budget_s, 1)
+    nt = _cpu_threads()
+    many = cpu_rate(fn, items, units_per_item, budget_s, nt)
+    return {"value": round(one, 4), "unit": unit, "cores": 1, "kind": "port", "sample": sample,
+            "threaded": {"value": round(many, 4), "cores": nt}}
+
+
+def spatial_secondary(af, ev, stream, reps=10, cpu=True):
     """FP32 spatial configurations (reported beside the headline): the
     reference CPU-backend semantics, timed per call with HIP events on the
     library stream; HBM roofline from algorithmic bytes (each input and output
-    element once)."""
+    element once); the oracle's C loops on the host beside each."""
     import oracle
 
     rng = np.random.default_rng(20261202 + 1)
@@ -342,35 +417,55 @@ def spatial_secondary(af, ev, stream, reps=10):
             d.update(extra)
         res[name] = d
 
+    nt = _cpu_threads()
     # 2a: 256 x 1024^2, 7x7, AF_BATCH_LHS
-    x = af.Array(rng.random((1024, 1024, 256), dtype=np.float32))
+    xh = rng.random((1024, 1024, 256), dtype=np.float32)
+    x = af.Array(xh)
     w7 = rng.standard_normal((7, 7)).astype(np.float32)
     W7 = af.Array(w7)
     ms = timeit(lambda: af.convolve2(x, W7, af.AF_CONV_DEFAULT, af.AF_CONV_SPATIAL))
     n = 1024 * 1024 * 256
-    entry("2a_conv2_lhs_256x1024sq_7x7", ms, 8.0 * n + 4 * 49, 2.0 * 49 * n, n)
+    imgs = [np.ascontiguousarray(xh[:, :, k]) for k in range(nt)]
+    entry("2a_conv2_lhs_256x1024sq_7x7", ms, 8.0 * n + 4 * 49, 2.0 * 49 * n, n, None if not cpu else {
+        "cpu_baseline": cpu_line(lambda im: oracle.convolve(im, w7, 2, False), imgs, 1024 * 1024 / 1e9, "GPix/s",
+                                 "1024x1024 slices of the same batch, 7x7 DEFAULT, the oracle's C loops")})
     # 2b: separable 17+17 Gaussian sigma 3 on the same images
     g = oracle.gaussian_kernel(17, 1, 3.0).ravel()
     G = af.Array(g)
     ms = timeit(lambda: af.convolve_sep(G, G, x, af.AF_CONV_DEFAULT))
-    entry("2b_sep_17x17_256x1024sq", ms, 8.0 * n + 8 * 17, 2.0 * 34 * n, n)
-    del x
+    entry("2b_sep_17x17_256x1024sq", ms, 8.0 * n + 8 * 17, 2.0 * 34 * n, n, None if not cpu else {
+        "cpu_baseline": cpu_line(lambda im: oracle.convolve2_sep(g, g, im, False), imgs, 1024 * 1024 / 1e9,
+                                 "GPix/s", "1024x1024 slices of the same batch, 17+17 taps, the oracle's "
+                                           "two passes with the T-typed temp")})
+    del x, xh, imgs
     # 2c: 128^3 volume, 3x3x3
-    v = af.Array(rng.random((128, 128, 128), dtype=np.float32))
-    w3 = af.Array(rng.uniform(-1, 1, (3, 3, 3)).astype(np.float32))
+    vh = rng.random((128, 128, 128), dtype=np.float32)
+    v = af.Array(vh)
+    w3h = rng.uniform(-1, 1, (3, 3, 3)).astype(np.float32)
+    w3 = af.Array(w3h)
     ms = timeit(lambda: af.convolve3(v, w3, af.AF_CONV_DEFAULT, af.AF_CONV_SPATIAL))
     n3 = 128 ** 3
-    entry("2c_conv3_128cube_3x3x3", ms, 8.0 * n3, 2.0 * 27 * n3, n3, {"note": "16 MiB working set: L2/MALL resident"})
+    extra = {"note": "16 MiB working set: L2/MALL resident"}
+    if cpu:
+        extra["cpu_baseline"] = cpu_line(lambda im: oracle.convolve(im, w3h, 3, False), [vh] * nt, n3 / 1e9,
+                                         "GPix/s", "the whole 128^3 volume, 3x3x3 DEFAULT")
+    entry("2c_conv3_128cube_3x3x3", ms, 8.0 * n3, 2.0 * 27 * n3, n3, extra)
     # 1: 512^2, 5x5
-    s1 = af.Array(rng.random((512, 512), dtype=np.float32))
-    w5 = af.Array(rng.uniform(-1, 1, (5, 5)).astype(np.float32))
+    s1h = rng.random((512, 512), dtype=np.float32)
+    s1 = af.Array(s1h)
+    w5h = rng.uniform(-1, 1, (5, 5)).astype(np.float32)
+    w5 = af.Array(w5h)
     ms = timeit(lambda: af.convolve2(s1, w5, af.AF_CONV_DEFAULT, af.AF_CONV_SPATIAL))
     n1 = 512 * 512
-    entry("1_conv2_512sq_5x5", ms, 8.0 * n1, 2.0 * 25 * n1, n1, {"note": "2 MiB working set: cache resident"})
+    extra = {"note": "2 MiB working set: cache resident"}
+    if cpu:
+        extra["cpu_baseline"] = cpu_line(lambda im: oracle.convolve(im, w5h, 2, False), [s1h] * nt, n1 / 1e9,
+                                         "GPix/s", "the whole 512x512 image, 5x5 DEFAULT")
+    entry("1_conv2_512sq_5x5", ms, 8.0 * n1, 2.0 * 25 * n1, n1, extra)
     return res
 
 
-def ragged_secondary(af, ev, stream):
+def ragged_secondary(af, ev, stream, cpu=True):
     """cfg 5: 2000 independent spatial 2-D convolutions (committed list in
     tests/golden/ragged_problems.json) as one graph-captured persistent
     launch, next to looping af_convolve2 over the same problems."""
@@ -409,7 +504,19 @@ def ragged_secondary(af, ev, stream):
     ms_loop = ev.ms(e0, e1)
     loop_wall = (time.perf_counter() - t0) * 1e3
     del outs, plan
+    cpu_b = None
+    if cpu:
+        import oracle
+
+        host = [(rng.random((w_, h_), dtype=np.float32), rng.random((fw, fh), dtype=np.float32), m)
+                for w_, h_, fw, fh, m in probs[:64]]
+        fl = sum(2.0 * p_[1].size * (p_[0].shape[0] + (p_[1].shape[0] - 1 if p_[2] else 0))
+                 * (p_[0].shape[1] + (p_[1].shape[1] - 1 if p_[2] else 0)) for p_ in host) / len(host)
+        cpu_b = cpu_line(lambda t: oracle.convolve(t[0], t[1], 2, bool(t[2])), host, fl / 1e9, "GFLOP/s",
+                         "the first 64 problems of the committed list, one af_convolve2 each, the oracle's "
+                         "C loops")
     return {"problems": len(probs), "ms_graph": round(ms_graph, 3), "ms_per_call_loop": round(ms_loop, 3),
+            "cpu_baseline": cpu_b,
             "ms_per_call_loop_host_wall": round(loop_wall, 3), "speedup": round(ms_loop / ms_graph, 2),
             "TFLOP_s": round(flops / (ms_graph * 1e-3) / 1e12, 2),
             "GB_s": round(bytes_ / (ms_graph * 1e-3) / 1e9, 1),
@@ -417,6 +524,103 @@ def ragged_secondary(af, ev, stream):
             "valu_frac": round(flops / (ms_graph * 1e-3) / 1e12 / PEAK["fp32_valu_TFs"], 3),
             "algorithmic_flops": flops, "algorithmic_bytes": bytes_,
             "plan_setup_s_incl_autotune": round(setup_s, 2)}
+
+
+def precision_lines(af, ev, stream, x, w, dy, X32, W32, DY32, steps, warmup):
+    """The headline step with FP16 operands (f16 arrays, FP32 accumulation,
+    f16 results) and in split-BF16 mode (f32 arrays, hi + lo bfloat16
+    operands, three products, f32 results): TFLOP/s (algorithmic FLOPs, as
+    the headline) and rel-L2 against FP64 of the FP32 operands."""
+    yd = (XD[0], XD[1], WD[3], XD[3])
+    flops = 3 * nn_flops(XD, WD, yd)
+    res = {}
+    Xh, Wh, DYh = X32.astype(af.f16), W32.astype(af.f16), DY32.astype(af.f16)
+    for name, args, kw in [("fp16", (Xh, Wh, DYh, af.f16), {}),
+                           ("split_bf16", (X32, W32, DY32, af.f32), {"precision": af.PRECISION_SPLIT_BF16})]:
+        ms = time_af_api(af, ev, stream, *args, steps, warmup, **kw)
+        outs = af_step(af, *args, **kw)
+        errs = fp64_errors(x, w, dy, outs)
+        res[name] = {"tflops": round(flops / (ms * 1e-3) / 1e12, 3), "ms_per_step": round(ms, 4),
+                     "rel_l2_vs_fp64": {n: float(f"{e:.3e}") for n, e in zip(["fwd", "dgrad", "wgrad"], errs)}}
+    return res
+
+
+# the stride-2 gradient layer: x [28,28,128,16], w [3,3,128,128], s2 p1 d1
+S2_XD = (28, 28, 128, 16)
+S2_WD = (3, 3, 128, 128)
+S2_GEOM = ((2, 2), (1, 1), (1, 1))
+
+
+def stride2_line(af, ev, stream, reps=20, cpu=True):
+    """Forward, data gradient and filter gradient of the stride-2 layer, FP32
+    and BF16 (bf16 arrays, FP32 accumulation), each pass timed alone through
+    the af_* API; rel-L2 of the BF16 passes against FP64."""
+    import oracle
+
+    rng = np.random.default_rng(20261202 + 3)
+    x = rng.standard_normal(S2_XD).astype(np.float32)
+    w = (rng.standard_normal(S2_WD) * np.sqrt(2.0 / (S2_WD[2] * 9))).astype(np.float32)
+    st, pd, dl = S2_GEOM
+    yd = (14, 14, S2_WD[3], S2_XD[3])
+    dy = rng.standard_normal(yd).astype(np.float32)
+    fl = nn_flops(S2_XD, S2_WD, yd)
+    X32, W32, DY32 = af.Array(x), af.Array(w), af.Array(dy)
+    out = {"workload": "x [28,28,128,16], w [3,3,128,128], stride 2, pad 1; dy [14,14,128,16]",
+           "flops_per_pass": fl}
+    for tag, (X, W, DY, ot) in {"fp32": (X32, W32, DY32, af.f32),
+                                "bf16": (X32.astype(af.bf16), W32.astype(af.bf16), DY32.astype(af.bf16),
+                                         af.bf16)}.items():
+        y = af.convolve2NN(X, W, st, pd, dl)
+        calls = {
+            "fwd": lambda: af.convolve2NN(X, W, st, pd, dl),
+            "dgrad": lambda: af.convolve2GradientNN(DY, X, W, y, st, pd, dl, af.AF_CONV_GRADIENT_DATA),
+            "wgrad": lambda: af.convolve2GradientNN(DY, X, W, y, st, pd, dl, af.AF_CONV_GRADIENT_FILTER),
+        }
+        peak = PEAK["bf16_TFs"] if tag == "bf16" else PEAK["fp32_TFs"]
+        line = {}
+        for pname, fn in calls.items():
+            hold = fn()
+            af.sync()
+            e0, e1 = ev.create(2)
+            ev.record(e0, stream)
+            for _ in range(reps):
+                hold = fn()
+            ev.record(e1, stream)
+            ms = ev.ms(e0, e1) / reps
+            tf = fl / (ms * 1e-3) / 1e12
+            line[pname] = {"us": round(ms * 1e3, 2), "TFLOP_s": round(tf, 2), "roofline_frac": round(tf / peak, 4)}
+            if tag == "bf16":
+                got = _as_f64(hold)
+                ref = (oracle.conv2_nn_direct_f64(x, w, st, pd, dl) if pname == "fwd" else
+                       oracle.conv2_data_grad_direct_f64(dy, x, w, st, pd, dl) if pname == "dgrad" else
+                       oracle.conv2_filter_grad_direct_f64(dy, x, w, st, pd, dl))
+                line[pname]["rel_l2_vs_fp64"] = float(f"{oracle.rel_l2(got, ref):.3e}")
+            del hold
+        tot = sum(v["us"] for v in line.values())
+        line["step_TFLOP_s"] = round(3 * fl / (tot * 1e-6) / 1e12, 2)
+        out[tag] = line
+    if cpu:
+        n = 2
+        xs, dys = x[..., :n], dy[..., :n]
+        sample = (xs, dys)
+
+        def one(t):
+            oracle.conv2_nn(t[0], w, st, pd, dl)
+            oracle.conv2_data_grad(t[1], t[0], w, st, pd, dl)
+            oracle.conv2_filter_grad(t[1], t[0], w, st, pd, dl)
+        r = cpu_rate(one, [sample], 3 * fl * n / S2_XD[3] / 1e12, 3.0, 1)
+        out["cpu_baseline"] = {"value": round(r, 4), "unit": "TFLOP/s", "cores": _blas_threads(), "kind": "port",
+                               "sample": f"FP32 fwd+dgrad+wgrad on N={n} images of the layer; im2col/col2im "
+                                         "single-threaded C, SGEMM on numpy's OpenBLAS"}
+    return out
+
+
+def _blas_threads():
+    try:
+        from threadpoolctl import threadpool_info
+        return int(max([i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"] or [1]))
+    except Exception:  # pragma: no cover
+        return os.cpu_count() or 1
 
 
 def main():
@@ -541,9 +745,17 @@ def main():
                                                          zip(names, errs)},
         "roofline": roof,
     }
+    out["cache_state"] = ("warm: the step's ~90 MB working set stays in the 256 MB MALL between steps; "
+                          "ms_per_step_cold_mall flushes it with a 1 GiB fill before each timed step")
+    out["ms_per_step_cold_mall"] = round(time_cold(af, ev, stream, Xb, Wb, DYb, af.bf16), 4)
+    out["host_dispatch_us_per_call"] = host_dispatch_us(af)
+    if not args.no_check:
+        out["precision_modes"] = precision_lines(af, ev, stream, x, w, dy, X32, W32, DY32,
+                                                 max(3, args.steps // 2), max(1, args.warmup // 2))
+    out["stride2_layer"] = stride2_line(af, ev, stream, cpu=not args.no_cpu)
     if not args.no_spatial:
-        out["secondary_fp32_spatial"] = spatial_secondary(af, ev, stream)
-        out["secondary_fp32_ragged"] = ragged_secondary(af, ev, stream)
+        out["secondary_fp32_spatial"] = spatial_secondary(af, ev, stream, cpu=not args.no_cpu)
+        out["secondary_fp32_ragged"] = ragged_secondary(af, ev, stream, cpu=not args.no_cpu)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

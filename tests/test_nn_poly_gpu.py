@@ -40,6 +40,13 @@ def _f64(a):
     return v.astype(np.float64)
 
 
+@pytest.fixture(autouse=True)
+def _force_poly(monkeypatch):
+    # these layers are small: without the switch the library would choose the
+    # gather kernel for them (too few tiles per phase to fill the chip)
+    monkeypatch.setenv("AFHIP_NN_POLY", "1")
+
+
 @pytest.mark.parametrize("dt", ["f32", "bf16", "f16"])
 @pytest.mark.parametrize("g", POLY_GEOMS, ids=_id)
 def test_poly_data_gradient(af, g, dt, monkeypatch):
@@ -60,6 +67,7 @@ def test_poly_data_gradient(af, g, dt, monkeypatch):
     got = dx.to_numpy()
     assert oracle.rel_l2(got, ref) <= 1e-5
     if dt == "f32":
+        monkeypatch.delenv("AFHIP_NN_POLY")
         monkeypatch.setenv("AFHIP_NN_GATHER", "1")
         dx0 = af.convolve2GradientNN(DY, X, W, y, st, pd, (1, 1), af.AF_CONV_GRADIENT_DATA).to_numpy()
         assert oracle.rel_l2(got, dx0) <= 1e-5
