@@ -86,7 +86,8 @@ EXPORTED = [
     "afhip_k_unwrap", "afhip_k_wrap", "afhip_k_region_copy", "afhip_k_sub", "afhip_k_div_cols",
     "afhip_k_iir_workspace", "afhip_k_iir", "afhip_k_conv2_nn_patch_noalloc_limit",
     "afhip_k_conv2_nn_patch_noalloc", "afhip_k_conv2_nn_dgrad_poly_workspace",
-    "afhip_k_conv2_nn_dgrad_poly",
+    "afhip_k_conv2_nn_dgrad_poly", "afhip_k_conv2_nn_gemm_workspace", "afhip_k_conv2_nn_gemm",
+    "afhip_k_conv2_nn_patch_tiles",
 ]
 
 
@@ -176,6 +177,11 @@ def lib() -> ctypes.CDLL:
         L.afhip_k_conv2_nn_dgrad_poly_workspace.argtypes = [ctypes.c_int] * 3 + [i64p] * 6
         L.afhip_k_conv2_nn_dgrad_poly_workspace.restype = ll
         L.afhip_k_conv2_nn_dgrad_poly.argtypes = [ctypes.c_int] * 3 + [vp, vp, vp] + [i64p] * 6 + [vp, ll, vp]
+        L.afhip_k_conv2_nn_gemm_workspace.argtypes = [ctypes.c_int] + [i64p] * 6
+        L.afhip_k_conv2_nn_gemm_workspace.restype = ll
+        L.afhip_k_conv2_nn_gemm.argtypes = [ctypes.c_int] * 4 + [vp, vp, vp] + [i64p] * 6 + [vp, ll, vp]
+        L.afhip_k_conv2_nn_patch_tiles.argtypes = [ctypes.c_int] * 3 + [i64p] * 6
+        L.afhip_k_conv2_nn_patch_tiles.restype = ll
         L.afhip_k_conv2_nn_patch_noalloc.argtypes = [ctypes.c_int] * 4 + [vp, vp, vp] + [i64p] * 6 + [vp]
         L.afhip_k_conv2_nn_patch_noalloc_limit.argtypes = []
         L.afhip_k_conv2_nn_patch_noalloc_limit.restype = ll

@@ -200,6 +200,34 @@ static void check_nn_geom(const NNGeom &g) {
 // gradient when the input patch fits in LDS, the row-staged kernel the filter
 // gradient of 16-bit operands; AFHIP_NN_GATHER=1 forces the gather kernels
 // (the tests use it to cover both paths).
+static int device_cus() {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+        return 256;
+    return n;
+}
+
+// Runs the split gather GEMM (afhip_k_conv2_nn_gemm) when the patch kernels
+// would launch fewer workgroups than a quarter of the CUs and the GEMM splits
+// its reduction across them (small or strided layers). AFHIP_NN_SMALL=0 in
+// the environment keeps such layers on the patch kernels.
+static bool run_small(int pass, const ArrayImpl &in, const ArrayImpl &w, af_dtype out_type, int precision,
+                      void *out, const NNGeom &g) {
+    const char *env = std::getenv("AFHIP_NN_SMALL");
+    if (env && env[0] == '0') return false;
+    const long long tiles = afhip_k_conv2_nn_patch_tiles(pass, in.type, precision, g.x.d, g.w.d, g.y.d, g.st,
+                                                         g.pd, g.dl);
+    if (tiles > 0 && tiles * 4 >= device_cus()) return false;
+    const long long need = afhip_k_conv2_nn_gemm_workspace(pass, g.x.d, g.w.d, g.y.d, g.st, g.pd, g.dl);
+    if (need <= 0) return false;
+    auto ws = scratch((size_t)need);
+    check_k(afhip_k_conv2_nn_gemm(pass, in.type, out_type, precision, out, in.data(), w.data(), g.x.d, g.w.d,
+                                  g.y.d, g.st, g.pd, g.dl, ws->ptr, need, stream()),
+            pass == 0 ? "convolve2_nn" : "convolve2_gradient_nn");
+    return true;
+}
+
 static bool run_patch(int pass, const ArrayImpl &in, const ArrayImpl &w, af_dtype out_type,
                       int precision, void *out, const NNGeom &g) {
     const char *env = std::getenv("AFHIP_NN_GATHER");
@@ -207,11 +235,24 @@ static bool run_patch(int pass, const ArrayImpl &in, const ArrayImpl &w, af_dtyp
     const long long need = afhip_k_conv2_nn_patch_workspace(pass, in.type, precision, g.x.d, g.w.d,
                                                             g.y.d, g.st, g.pd, g.dl);
     if (need <= 0) return false;
+    if (run_small(pass, in, w, out_type, precision, out, g)) return true;
     auto ws = scratch((size_t)need);
     check_k(afhip_k_conv2_nn_patch(pass, in.type, out_type, precision, out, in.data(), w.data(), g.x.d,
                                    g.w.d, g.y.d, g.st, g.pd, g.dl, ws->ptr, need, stream()),
             pass == 0 ? "convolve2_nn" : "convolve2_gradient_nn");
     return true;
+}
+
+// The gather GEMM for every geometry, its reduction split when that fills
+// the chip (workspace from the pool).
+static void run_gemm(int pass, const ArrayImpl &in, const ArrayImpl &w, af_dtype out_type, int precision,
+                     void *out, const NNGeom &g) {
+    const long long need = afhip_k_conv2_nn_gemm_workspace(pass, g.x.d, g.w.d, g.y.d, g.st, g.pd, g.dl);
+    std::shared_ptr<Buffer> ws;
+    if (need > 0) ws = scratch((size_t)need);
+    check_k(afhip_k_conv2_nn_gemm(pass, in.type, out_type, precision, out, in.data(), w.data(), g.x.d, g.w.d,
+                                  g.y.d, g.st, g.pd, g.dl, ws ? ws->ptr : nullptr, need, stream()),
+            pass == 0 ? "convolve2_nn" : "convolve2_gradient_nn");
 }
 
 static af_dtype nn_out_type(af_dtype in, af_dtype requested) {
@@ -231,9 +272,7 @@ static af_array conv2_nn(const ArrayImpl &x, const ArrayImpl &w, const NNGeom &g
         return out.release();
     }
     if (run_patch(0, x, w, out_type, precision, out.data(), g)) return out.release();
-    check_k(afhip_k_conv2_nn_fwd(x.type, out_type, precision, out.data(), x.data(), w.data(), g.x.d,
-                                 g.w.d, g.y.d, g.st, g.pd, g.dl, stream()),
-            "convolve2_nn");
+    run_gemm(0, x, w, out_type, precision, out.data(), g);
     return out.release();
 }
 
@@ -255,7 +294,9 @@ static af_array conv2_data_grad(const ArrayImpl &dy, const ArrayImpl &x, const A
     }
     if (run_patch(1, dy, w, out_type, precision, o, g)) return out.release();
     // strided layers: the polyphase split into stride-1 convolutions on the
-    // patch kernels (AFHIP_NN_GATHER=1 keeps the gather kernel)
+    // patch kernels when its phases fill the chip, else the split gather
+    // GEMM over phase-ordered pixels (AFHIP_NN_GATHER=1 keeps the gather
+    // kernel)
     const char *gather = std::getenv("AFHIP_NN_GATHER");
     if (!(gather && gather[0] == '1')) {
         const long long pb = afhip_k_conv2_nn_dgrad_poly_workspace(x.type, out_type, precision, g.x.d, g.w.d,
@@ -268,9 +309,7 @@ static af_array conv2_data_grad(const ArrayImpl &dy, const ArrayImpl &x, const A
             return out.release();
         }
     }
-    check_k(afhip_k_conv2_nn_dgrad(x.type, out_type, precision, o, dy.data(), w.data(), g.x.d, g.w.d,
-                                   g.y.d, g.st, g.pd, g.dl, stream()),
-            "convolve2_gradient_nn");
+    run_gemm(1, dy, w, out_type, precision, o, g);
     return out.release();
 }
 

@@ -74,6 +74,19 @@ inline int launch_status() {
 // kernels cap their grid here.
 constexpr int kMaxGridBlocks = 256 * 64;
 
+// Compute units of the current device (256 on MI355X), cached per device.
+static inline int cu_count_current() {
+    static int cus[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (cus[dev] <= 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cus[dev] = n;
+    }
+    return cus[dev];
+}
+
 // Remaps a linear workgroup id so that each of the 8 XCDs (which receive
 // workgroups round-robin) walks a contiguous range of tiles; neighbouring
 // tiles then share halo lines in that XCD's L2.

@@ -12,8 +12,12 @@
 //   fwd   M = K,  Ncol = N*OH*OW, R = (tap, C)   A = w,  B = im2col(x)
 //   dgrad M = C,  Ncol = N*H*W,   R = (tap, K)   A = w,  B = col(dy) gathered
 //                                                through the stride phases
-//   wgrad M = K,  Ncol = C*fh*fw, R = N*OH*OW    A = dy, B = im2col(x)^T,
-//                 split along R with a deterministic second-stage sum.
+//   wgrad M = K,  Ncol = C*fh*fw, R = N*OH*OW    A = dy, B = im2col(x)^T.
+// Layers with too few output tiles to fill the chip split R across
+// workgroups (FP32 partial tiles, then a second pass that sums them in split
+// order: deterministic). A strided data gradient orders its columns by stride
+// phase (x mod s0, y mod s1), each phase padded to whole tiles, and visits
+// only the taps that reach that phase: about 1/(s0*s1) of the slices.
 // Nothing is materialised: each 32-deep slice of R is gathered from HBM into
 // registers, converted to the operand format and stored to LDS with R
 // contiguous per row, where the MFMA fragments read it with ds_read_b128.
@@ -26,6 +30,8 @@
 //   SPLIT hi = bf16(v), lo = bf16(v - hi); hi*hi' + hi*lo' + lo*hi' on BF16
 // Accumulation is FP32 in every mode. One LDS buffer; the next slice's global
 // loads are in flight in registers while the current slice is multiplied.
+#include <cstdlib>
+
 #include "kcommon.hpp"
 #include "../../../include/afhip_kernels.h"
 
@@ -39,6 +45,10 @@ typedef uint16_t u16x8_t __attribute__((ext_vector_type(8)));
 enum Mode : int { M_F32 = 0, M_BF16 = 1, M_F16 = 2, M_SPLIT = 3 };
 enum Pass : int { P_FWD = 0, P_DGRAD = 1, P_WGRAD = 2 };
 
+// Strided data gradients order their columns by stride phase when there are
+// at most this many phases (s0 * s1).
+constexpr int kMaxPhases = 16;
+
 struct NNParams {
     // x [W,H,C,N], w [fw,fh,C,K], y [OW,OH,K,N]
     int W, H, C, N, fw, fh, K, OW, OH;
@@ -47,11 +57,41 @@ struct NNParams {
     long long M, Ncol;   // GEMM extents
     int R;               // reduction extent (wgrad) -- informational
     int kchunks;         // 32-deep slices per tap (fwd: ceil(C/32), dgrad: ceil(K/32))
-    long long ksteps;    // total slices
+    long long ksteps;    // total slices (phased data gradient: the most of any phase)
     long long ksteps_per_split;
-    int nsplit;
+    int nsplit;          // > 1: FP32 partial tiles per split, summed by nn_split_reduce / wgrad_reduce
     long long tilesM, tilesN;
+    // Phased data gradient (stride > 1): the input pixels (x, y) with
+    // x = s0*mx + rx, y = s1*my + ry form phase ph = ry*s0 + rx; each phase's
+    // N*My*Mx columns start at column tile ph_tile0[ph] and only the taps
+    // (i, j) with (rx + p0 - i*d0) and (ry + p1 - j*d1) divisible by the
+    // strides meet them, so only those taps are visited.
+    int phased;
+    int nph;
+    int ph_tile0[kMaxPhases + 1];
 };
+
+// Phase extents: pixels of residue r along an axis of length L and stride s,
+// and the taps of a filter (length f, dilation d, padding pd) that reach them.
+__host__ __device__ inline int phase_len(int L, int s, int r) { return r < L ? (L - r + s - 1) / s : 0; }
+__host__ __device__ inline bool phase_tap(int r, int pd, int i, int d, int s) {
+    const int v = (r + pd - i * d) % s;
+    return v == 0;
+}
+__host__ __device__ inline int phase_taps(int f, int r, int pd, int d, int s) {
+    int n = 0;
+    for (int i = 0; i < f; ++i) n += phase_tap(r, pd, i, d, s) ? 1 : 0;
+    return n;
+}
+// The u-th tap (in increasing order) that reaches residue r.
+__host__ __device__ inline int phase_tap_at(int f, int r, int pd, int d, int s, int u) {
+    for (int i = 0; i < f; ++i)
+        if (phase_tap(r, pd, i, d, s)) {
+            if (u == 0) return i;
+            --u;
+        }
+    return 0;
+}
 
 constexpr int BM = 128, BN = 128, BK = 32;
 constexpr int NT = 256;
@@ -254,7 +294,7 @@ struct Gather<TIn, P_FWD> {
     long long arow, brow;  // global row indices
     bool bvalid;
     Pix bp;
-    __device__ Gather(const TIn *x_, const TIn *w_, const NNParams &p_, long long ar, long long br)
+    __device__ Gather(const TIn *x_, const TIn *w_, const NNParams &p_, long long ar, long long br, int)
         : x(x_), w(w_), p(p_), arow(ar), brow(br) {
         bvalid = br < p.Ncol;
         bp = decode(bvalid ? br : 0, p.OH, p.OW);
@@ -291,18 +331,45 @@ template <typename TIn>
 struct Gather<TIn, P_DGRAD> {
     const TIn *dy, *w;
     const NNParams &p;
-    long long arow, brow;
+    long long arow;
     bool bvalid;
-    Pix bp;
-    __device__ Gather(const TIn *dy_, const TIn *w_, const NNParams &p_, long long ar, long long br)
-        : dy(dy_), w(w_), p(p_), arow(ar), brow(br) {
-        bvalid = br < p.Ncol;
-        bp = decode(bvalid ? br : 0, p.H, p.W);
+    int bn, by, bx;     // this lane's B row: image and input pixel
+    int rx, ry, Ti;     // phased: the tile's phase and its tap count along W
+    __device__ Gather(const TIn *dy_, const TIn *w_, const NNParams &p_, long long ar, long long br, int ph)
+        : dy(dy_), w(w_), p(p_), arow(ar) {
+        if (p.phased) {
+            rx = ph % p.s0;
+            ry = ph / p.s0;
+            const int Mx = phase_len(p.W, p.s0, rx), My = phase_len(p.H, p.s1, ry);
+            bvalid = br < (long long)p.N * My * Mx;
+            const Pix q = decode(bvalid ? br : 0, My, Mx);
+            bn = q.n;
+            by = p.s1 * q.a + ry;
+            bx = p.s0 * q.b + rx;
+            Ti = phase_taps(p.fw, rx, p.p0, p.d0, p.s0);
+        } else {
+            rx = ry = 0;
+            Ti = p.fw;
+            bvalid = br < p.Ncol;
+            const Pix q = decode(bvalid ? br : 0, p.H, p.W);
+            bn = q.n;
+            by = q.a;
+            bx = q.b;
+        }
     }
     __device__ void load(long long ks, int kk0, float (&va)[16], float (&vb)[16]) const {
-        const int t = (int)(ks / p.kchunks);
-        const int k0 = (int)(ks - (long long)t * p.kchunks) * BK + kk0;
-        const int i = t % p.fw, j = t / p.fw;
+        const int u = (int)(ks / p.kchunks);
+        const int k0 = (int)(ks - (long long)u * p.kchunks) * BK + kk0;
+        int i, j;
+        if (p.phased) {
+            const int ui = u % Ti;
+            i = phase_tap_at(p.fw, rx, p.p0, p.d0, p.s0, ui);
+            j = phase_tap_at(p.fh, ry, p.p1, p.d1, p.s1, u / Ti);
+        } else {
+            i = u % p.fw;
+            j = u / p.fw;
+        }
+        const int t = j * p.fw + i;
         // A: w[k][c][fh-1-j][fw-1-i], row = c, R = k
         const bool ac = arow < p.C;
         const long long crs = (long long)p.C * p.RS;
@@ -312,14 +379,14 @@ struct Gather<TIn, P_DGRAD> {
             const int k = k0 + e;
             va[e] = (ac && k < p.K) ? ld_f(w, abase + (long long)k * crs) : 0.f;
         }
-        // B: dy[n][k][oh][ow] where oh*s1 - p1 + j*d1 = h, ow*s0 - p0 + i*d0 = w
-        const int nh = bp.a + p.p1 - j * p.d1;
-        const int nw = bp.b + p.p0 - i * p.d0;
+        // B: dy[n][k][oh][ow] where oh*s1 - p1 + j*d1 = y, ow*s0 - p0 + i*d0 = x
+        const int nh = by + p.p1 - j * p.d1;
+        const int nw = bx + p.p0 - i * p.d0;
         bool ok = bvalid && nh >= 0 && nw >= 0 && (nh % p.s1) == 0 && (nw % p.s0) == 0;
         const int oh = ok ? nh / p.s1 : 0, ow = ok ? nw / p.s0 : 0;
         ok = ok && oh < p.OH && ow < p.OW;
         const long long ohw = (long long)p.OH * p.OW;
-        const long long bbase = ((long long)bp.n * p.K) * ohw + (long long)oh * p.OW + ow;
+        const long long bbase = ((long long)bn * p.K) * ohw + (long long)oh * p.OW + ow;
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
             const int k = k0 + e;
@@ -336,7 +403,7 @@ struct Gather<TIn, P_WGRAD> {
     long long arow, brow;
     int bc, bi, bj;  // B row -> channel and tap (i along W, j along H)
     bool bvalid;
-    __device__ Gather(const TIn *x_, const TIn *dy_, const NNParams &p_, long long ar, long long br)
+    __device__ Gather(const TIn *x_, const TIn *dy_, const NNParams &p_, long long ar, long long br, int)
         : x(x_), dy(dy_), p(p_), arow(ar), brow(br) {
         bvalid = br < p.Ncol;
         const int r = bvalid ? (int)br : 0;
@@ -389,13 +456,25 @@ __global__ __launch_bounds__(NT) void nn_gemm(TOut *__restrict__ out, const TIn 
     const long long split = bid / tiles;
     const long long tl = bid - split * tiles;
     const long long tm = tl / p.tilesN, tn = tl - tm * p.tilesN;
-    const long long m0 = tm * BM, n0 = tn * BN;
+    const long long m0 = tm * BM;
+    // phased data gradient: the tile's phase and its column offset in it
+    int ph = 0;
+    long long n0 = tn * BN;
+    long long ksteps = p.ksteps, kps = p.ksteps_per_split;
+    if (PASS == P_DGRAD && p.phased) {
+        while (ph + 1 < p.nph && tn >= p.ph_tile0[ph + 1]) ++ph;
+        n0 = (tn - p.ph_tile0[ph]) * BN;
+        const int rx = ph % p.s0, ry = ph / p.s0;
+        ksteps = (long long)phase_taps(p.fw, rx, p.p0, p.d0, p.s0) * phase_taps(p.fh, ry, p.p1, p.d1, p.s1) *
+                 p.kchunks;
+        kps = (ksteps + p.nsplit - 1) / p.nsplit;
+    }
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int lrow = tid & 127, kk0 = (tid >> 7) * 16;
 
-    Gather<TIn, PASS> g(in0, in1, p, m0 + lrow, n0 + lrow);
+    Gather<TIn, PASS> g(in0, in1, p, m0 + lrow, n0 + lrow, ph);
 
     f32x4_t acc[4][4];
 #pragma unroll
@@ -403,9 +482,9 @@ __global__ __launch_bounds__(NT) void nn_gemm(TOut *__restrict__ out, const TIn 
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
 
-    const long long ks0 = split * p.ksteps_per_split;
-    long long ks1 = ks0 + p.ksteps_per_split;
-    if (ks1 > p.ksteps) ks1 = p.ksteps;
+    const long long ks0 = split * kps;
+    long long ks1 = ks0 + kps;
+    if (ks1 > ksteps) ks1 = ksteps;
 
     float va[16], vb[16];
     if (ks0 < ks1) g.load(ks0, kk0, va, vb);
@@ -420,24 +499,52 @@ __global__ __launch_bounds__(NT) void nn_gemm(TOut *__restrict__ out, const TIn 
 
     // Epilogue. acc[i][j][v] is C[row][col] with
     // row = m0 + wm*64 + i*16 + 4*(lane/16) + v, col = n0 + wn*64 + j*16 + lane%16.
+    // Split reductions store FP32 partial tiles (row-major over the padded
+    // columns; nn_split_reduce / wgrad_reduce sum them in split order).
     const int g4 = (lane >> 4) * 4, c16 = lane & 15;
+    if (p.nsplit > 1) {
+        const long long pc0 = (PASS == P_DGRAD && p.phased) ? (long long)p.ph_tile0[ph] * BN : 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const long long col = n0 + wn * 64 + j * 16 + c16;
+            if (pc0 + col >= p.Ncol) continue;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const long long row = m0 + wm * 64 + i * 16 + g4 + v;
+                    if (row < p.M) partial[(split * p.M + row) * p.Ncol + pc0 + col] = acc[i][j][v];
+                }
+        }
+        return;
+    }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const long long col = n0 + wn * 64 + j * 16 + c16;
-        if (col >= p.Ncol) continue;
         long long cbase;
         long long rstride;
         if constexpr (PASS == P_FWD) {
+            if (col >= p.Ncol) continue;
             const Pix q = decode(col, p.OH, p.OW);
             const long long ohw = (long long)p.OH * p.OW;
             cbase = (long long)q.n * p.K * ohw + (long long)q.a * p.OW + q.b;
             rstride = ohw;
         } else if constexpr (PASS == P_DGRAD) {
-            const Pix q = decode(col, p.H, p.W);
             const long long hw = (long long)p.H * p.W;
-            cbase = (long long)q.n * p.C * hw + (long long)q.a * p.W + q.b;
+            if (p.phased) {
+                const int rx = ph % p.s0, ry = ph / p.s0;
+                const int Mx = phase_len(p.W, p.s0, rx), My = phase_len(p.H, p.s1, ry);
+                if (col >= (long long)p.N * My * Mx) continue;
+                const Pix q = decode(col, My, Mx);
+                cbase = (long long)q.n * p.C * hw + (long long)(p.s1 * q.a + ry) * p.W + (p.s0 * q.b + rx);
+            } else {
+                if (col >= p.Ncol) continue;
+                const Pix q = decode(col, p.H, p.W);
+                cbase = (long long)q.n * p.C * hw + (long long)q.a * p.W + q.b;
+            }
             rstride = hw;
         } else {
+            if (col >= p.Ncol) continue;
             // dW[fw-1-i, fh-1-j, c, k]: element (k*C + c)*RS + RS-1-t
             const int c = (int)(col / p.RS);
             const int t = (int)(col - (long long)c * p.RS);
@@ -450,13 +557,46 @@ __global__ __launch_bounds__(NT) void nn_gemm(TOut *__restrict__ out, const TIn 
             for (int v = 0; v < 4; ++v) {
                 const long long row = m0 + wm * 64 + i * 16 + g4 + v;
                 if (row >= p.M) continue;
-                const float val = acc[i][j][v];
-                if (PASS == P_WGRAD && p.nsplit > 1) {
-                    partial[(split * p.M + row) * p.Ncol + col] = val;
-                } else {
-                    st_f(out, cbase + row * rstride, val);
-                }
+                st_f(out, cbase + row * rstride, acc[i][j][v]);
             }
+    }
+}
+
+// Second stage of a split forward or data gradient: sums the nsplit partial
+// tiles of each output element in split order (deterministic) and stores it
+// in the output layout. Padding columns of the phased layout are skipped.
+template <typename TOut, int PASS>
+__global__ __launch_bounds__(256) void nn_split_reduce(TOut *__restrict__ out, const float *__restrict__ partial,
+                                                       NNParams p) {
+    const long long total = p.M * p.Ncol;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const long long row = e / p.Ncol, col = e - row * p.Ncol;
+        long long dst;
+        if constexpr (PASS == P_FWD) {
+            const Pix q = decode(col, p.OH, p.OW);
+            const long long ohw = (long long)p.OH * p.OW;
+            dst = ((long long)q.n * p.K + row) * ohw + (long long)q.a * p.OW + q.b;
+        } else {
+            const long long hw = (long long)p.H * p.W;
+            if (p.phased) {
+                const long long tn = col / BN;
+                int ph = 0;
+                while (ph + 1 < p.nph && tn >= p.ph_tile0[ph + 1]) ++ph;
+                const long long lc = col - (long long)p.ph_tile0[ph] * BN;
+                const int rx = ph % p.s0, ry = ph / p.s0;
+                const int Mx = phase_len(p.W, p.s0, rx), My = phase_len(p.H, p.s1, ry);
+                if (lc >= (long long)p.N * My * Mx) continue;
+                const Pix q = decode(lc, My, Mx);
+                dst = ((long long)q.n * p.C + row) * hw + (long long)(p.s1 * q.a + ry) * p.W + (p.s0 * q.b + rx);
+            } else {
+                const Pix q = decode(col, p.H, p.W);
+                dst = ((long long)q.n * p.C + row) * hw + (long long)q.a * p.W + q.b;
+            }
+        }
+        float s = 0.f;
+        for (int sp = 0; sp < p.nsplit; ++sp) s += partial[sp * total + e];
+        st_f(out, dst, s);
     }
 }
 
@@ -501,6 +641,9 @@ static int setup(NNParams &p, int pass, const long long *xd, const long long *wd
     if (p.s0 < 1 || p.s1 < 1 || p.d0 < 1 || p.d1 < 1) return K_ERR_ARG;
     p.RS = p.fw * p.fh;
     p.nsplit = 1;
+    p.phased = 0;
+    p.nph = 1;
+    for (int i = 0; i <= kMaxPhases; ++i) p.ph_tile0[i] = 0;
     if (pass == P_FWD) {
         p.M = p.K;
         p.Ncol = (long long)p.N * p.OH * p.OW;
@@ -511,6 +654,27 @@ static int setup(NNParams &p, int pass, const long long *xd, const long long *wd
         p.Ncol = (long long)p.N * p.H * p.W;
         p.kchunks = (p.K + BK - 1) / BK;
         p.ksteps = (long long)p.RS * p.kchunks;
+        // AFHIP_NN_PHASED=0 in the environment keeps the unphased order
+        const char *env = std::getenv("AFHIP_NN_PHASED");
+        const bool off = env && env[0] == '0';
+        if (!off && (p.s0 > 1 || p.s1 > 1) && p.s0 * p.s1 <= kMaxPhases) {
+            p.phased = 1;
+            p.nph = p.s0 * p.s1;
+            long long t0 = 0, kmax = 0;
+            for (int ph = 0; ph < p.nph; ++ph) {
+                const int rx = ph % p.s0, ry = ph / p.s0;
+                const long long cols = (long long)p.N * phase_len(p.H, p.s1, ry) * phase_len(p.W, p.s0, rx);
+                p.ph_tile0[ph] = (int)t0;
+                t0 += (cols + BN - 1) / BN;
+                const long long ks = (long long)phase_taps(p.fw, rx, p.p0, p.d0, p.s0) *
+                                     phase_taps(p.fh, ry, p.p1, p.d1, p.s1) * p.kchunks;
+                if (ks > kmax) kmax = ks;
+            }
+            if (t0 >= (1LL << 31)) return K_ERR_SIZE;
+            p.ph_tile0[p.nph] = (int)t0;
+            p.Ncol = t0 * BN;
+            p.ksteps = kmax;
+        }
     } else {
         p.M = p.K;
         p.Ncol = (long long)p.C * p.RS;
@@ -535,18 +699,49 @@ static int wgrad_splits(const NNParams &p) {
     return (int)want;
 }
 
+// Split count for the forward and data gradient: layers whose output tiles
+// leave most CUs idle split the reduction (FP32 partial tiles, a second
+// summing pass), about two workgroups per CU, each with at least 4 slices,
+// the partials within 256 MiB. AFHIP_NN_SPLIT=<n> in the environment forces
+// n (1 = unsplit).
+static int gemm_splits(const NNParams &p) {
+    const long long tiles = p.tilesM * p.tilesN;
+    long long want;
+    const char *env = std::getenv("AFHIP_NN_SPLIT");
+    if (env && env[0] >= '1' && env[0] <= '9') {
+        want = std::atoll(env);
+    } else {
+        const long long cus = cu_count_current();
+        if (tiles * 2 >= cus) return 1;
+        want = (2 * cus + tiles - 1) / tiles;
+        const long long maxs = p.ksteps / 4;
+        if (want > maxs) want = maxs;
+    }
+    while (want > 1 && want * p.M * p.Ncol * 4 > (256LL << 20)) --want;
+    if (want > p.ksteps) want = p.ksteps;
+    if (want < 1) want = 1;
+    return (int)want;
+}
+
 template <int MODE, typename TIn, typename TOut>
 static int run(int pass, void *out, const void *a0, const void *a1, NNParams &p, void *ws,
                long long ws_bytes, hipStream_t s) {
     if (p.M == 0 || p.Ncol == 0) return K_OK;
     const int smem = Lds<MODE>::BYTES;
     const long long tiles = p.tilesM * p.tilesN;
-    if (pass == P_FWD) {
-        nn_gemm<MODE, TIn, TOut, P_FWD><<<dim3((unsigned)tiles), NT, smem, s>>>(
-            (TOut *)out, (const TIn *)a0, (const TIn *)a1, nullptr, p);
-    } else if (pass == P_DGRAD) {
-        nn_gemm<MODE, TIn, TOut, P_DGRAD><<<dim3((unsigned)tiles), NT, smem, s>>>(
-            (TOut *)out, (const TIn *)a0, (const TIn *)a1, nullptr, p);
+    if (pass == P_FWD || pass == P_DGRAD) {
+        p.nsplit = gemm_splits(p);
+        if (p.nsplit > 1 && (ws == nullptr || ws_bytes < (long long)p.nsplit * p.M * p.Ncol * 4)) p.nsplit = 1;
+        p.ksteps_per_split = (p.ksteps + p.nsplit - 1) / p.nsplit;
+        auto fn = pass == P_FWD ? nn_gemm<MODE, TIn, TOut, P_FWD> : nn_gemm<MODE, TIn, TOut, P_DGRAD>;
+        fn<<<dim3((unsigned)(tiles * p.nsplit)), NT, smem, s>>>((TOut *)out, (const TIn *)a0, (const TIn *)a1,
+                                                               (float *)ws, p);
+        if (p.nsplit > 1) {
+            long long blocks = (p.M * p.Ncol + 255) / 256;
+            if (blocks > kMaxGridBlocks) blocks = kMaxGridBlocks;
+            auto red = pass == P_FWD ? nn_split_reduce<TOut, P_FWD> : nn_split_reduce<TOut, P_DGRAD>;
+            red<<<dim3((unsigned)blocks), 256, 0, s>>>((TOut *)out, (const float *)ws, p);
+        }
     } else {
         if (p.ksteps == 0) {
             // empty reduction: dW = 0
@@ -611,15 +806,37 @@ static int dispatch(int pass, int in_type, int out_type, int precision, void *ou
 
 using namespace afhip;
 
+extern "C" long long afhip_k_conv2_nn_gemm_workspace(int pass, const long long *xdims, const long long *wdims,
+                                                     const long long *ydims, const long long *stride,
+                                                     const long long *pad, const long long *dil) {
+    if (pass != P_FWD && pass != P_DGRAD) return 0;
+    NNParams p;
+    if (setup(p, pass, xdims, wdims, ydims, stride, pad, dil)) return 0;
+    if (p.M == 0 || p.Ncol == 0) return 0;
+    const int ns = gemm_splits(p);
+    return ns > 1 ? (long long)ns * p.M * p.Ncol * 4 : 0;
+}
+
+extern "C" int afhip_k_conv2_nn_gemm(int pass, int in_type, int out_type, int precision, void *out,
+                                     const void *in, const void *w, const long long *xdims,
+                                     const long long *wdims, const long long *ydims,
+                                     const long long *stride, const long long *pad, const long long *dil,
+                                     void *workspace, long long workspace_bytes, void *stream) {
+    if (pass != P_FWD && pass != P_DGRAD) return K_ERR_ARG;
+    NNParams p;
+    int rc = setup(p, pass, xdims, wdims, ydims, stride, pad, dil);
+    if (rc) return rc;
+    return dispatch(pass, in_type, out_type, precision, out, in, w, p, workspace, workspace_bytes,
+                    (hipStream_t)stream);
+}
+
 extern "C" int afhip_k_conv2_nn_fwd(int in_type, int out_type, int precision, void *y,
                                     const void *x, const void *w, const long long *xdims,
                                     const long long *wdims, const long long *ydims,
                                     const long long *stride, const long long *pad,
                                     const long long *dil, void *stream) {
-    NNParams p;
-    int rc = setup(p, P_FWD, xdims, wdims, ydims, stride, pad, dil);
-    if (rc) return rc;
-    return dispatch(P_FWD, in_type, out_type, precision, y, x, w, p, nullptr, 0, (hipStream_t)stream);
+    return afhip_k_conv2_nn_gemm(P_FWD, in_type, out_type, precision, y, x, w, xdims, wdims, ydims, stride, pad,
+                                 dil, nullptr, 0, stream);
 }
 
 extern "C" int afhip_k_conv2_nn_dgrad(int in_type, int out_type, int precision, void *dx,
@@ -627,10 +844,8 @@ extern "C" int afhip_k_conv2_nn_dgrad(int in_type, int out_type, int precision, 
                                       const long long *wdims, const long long *ydims,
                                       const long long *stride, const long long *pad,
                                       const long long *dil, void *stream) {
-    NNParams p;
-    int rc = setup(p, P_DGRAD, xdims, wdims, ydims, stride, pad, dil);
-    if (rc) return rc;
-    return dispatch(P_DGRAD, in_type, out_type, precision, dx, dy, w, p, nullptr, 0, (hipStream_t)stream);
+    return afhip_k_conv2_nn_gemm(P_DGRAD, in_type, out_type, precision, dx, dy, w, xdims, wdims, ydims, stride,
+                                 pad, dil, nullptr, 0, stream);
 }
 
 extern "C" long long afhip_k_conv2_nn_wgrad_workspace(const long long *xdims, const long long *wdims,

@@ -1558,6 +1558,48 @@ extern "C" long long afhip_k_conv2_nn_patch_workspace(int pass, int in_type, int
     return (long long)p.ktiles * p.nchunks * p.RS * P_BM * kc;
 }
 
+template <typename T>
+struct PTag {
+    using type = T;
+};
+
+extern "C" long long afhip_k_conv2_nn_patch_tiles(int pass, int in_type, int precision, const long long *xd,
+                                                  const long long *wd, const long long *yd, const long long *st,
+                                                  const long long *pd, const long long *dl) {
+    if (afhip_k_conv2_nn_patch_workspace(pass, in_type, precision, xd, wd, yd, st, pd, dl) <= 0) return 0;
+    const int mode = mode_of(in_type, precision);
+    PatchParams p;
+    fill(p, pass, xd, wd, yd, st, pd, dl);
+    auto tiles = [&](auto md, auto tin) -> long long {
+        constexpr int MD = decltype(md)::value;
+        using TIn = typename decltype(tin)::type;
+        plan<MD>(p);
+        if constexpr (MD != PM_SPLIT && sizeof(TIn) == PCfg<MD>::ESZ) {
+            if constexpr (MD == PM_BF16 || MD == PM_F16) {
+                WresParams wr;
+                int nch = 0, nq = 0;
+                const long long wsz = (long long)p.ktiles * p.nchunks * p.RS * P_BM * PCfg<MD>::KC * 2;
+                if (plan_wres<MD, TIn>(wr, nch, nq, p, nullptr, pass, wsz)) return wr.nunits;
+            }
+            PatchParams q;
+            int wm = 0;
+            if (plan_pipe<MD, TIn>(q, p, nullptr, wm)) return q.ntiles;
+        }
+        return p.ntiles;
+    };
+    using I = std::integral_constant<int, PM_F32>;
+    switch (mode) {
+        case PM_F32: return tiles(I{}, PTag<float>{});
+        case PM_BF16:
+            return in_type == DT_BF16 ? tiles(std::integral_constant<int, PM_BF16>{}, PTag<uint16_t>{})
+                                      : tiles(std::integral_constant<int, PM_BF16>{}, PTag<float>{});
+        case PM_F16:
+            return in_type == DT_F16 ? tiles(std::integral_constant<int, PM_F16>{}, PTag<_Float16>{})
+                                     : tiles(std::integral_constant<int, PM_F16>{}, PTag<float>{});
+        default: return tiles(std::integral_constant<int, PM_SPLIT>{}, PTag<float>{});
+    }
+}
+
 static int patch_call(int what, int pass, int in_type, int out_type, int precision, void *out,
                       const void *in, const void *w, const long long *xd, const long long *wd,
                       const long long *yd, const long long *st, const long long *pd,

@@ -601,7 +601,13 @@ static bool wg_plan(WGParams &p, int fw, int s0, int d0) {
     p.ktiles = (p.K + WG_BK - 1) / WG_BK;
     p.ctiles = (p.C + WG_BC - 1) / WG_BC;
     const int tiles = p.ktiles * p.ctiles;
-    int ns = (256 + tiles - 1) / tiles;
+    // Split count: one workgroup per CU (measured fastest on the benchmark
+    // layers, whose units cost far more than their partial tiles).
+    // AFHIP_NN_WGRAD_SPLIT=<n> in the environment forces n splits.
+    const int cus = cu_count_current();
+    int ns = (cus + tiles - 1) / tiles;
+    const char *env = std::getenv("AFHIP_NN_WGRAD_SPLIT");
+    if (env && env[0] >= '1' && env[0] <= '9') ns = std::atoi(env);
     if (ns > p.total_units) ns = p.total_units;
     if (ns < 1) ns = 1;
     p.upsplit = (p.total_units + ns - 1) / ns;

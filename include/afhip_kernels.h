@@ -120,6 +120,28 @@ AFHIP_KAPI int afhip_k_conv2_nn_dgrad(int in_type, int out_type, int precision,
                                       const long long *pad,
                                       const long long *dil, void *stream);
 
+/* The gather GEMM behind afhip_k_conv2_nn_fwd (pass 0: in = x, out = y) and
+ * afhip_k_conv2_nn_dgrad (pass 1: in = dy, out = dx), for any geometry, with
+ * a workspace: layers whose output tiles leave most CUs idle split the
+ * reduction into FP32 partial tiles that a second pass sums in a fixed order.
+ * Strided data gradients visit only the taps of each stride phase. The query
+ * returns the workspace bytes of the split (0: the call runs unsplit, and a
+ * NULL or short workspace also runs unsplit). Same dims and codes as
+ * afhip_k_conv2_nn_fwd/_dgrad (src/backend/cpu/convolve.cpp:123-211). */
+AFHIP_KAPI long long afhip_k_conv2_nn_gemm_workspace(
+    int pass, const long long *xdims, const long long *wdims,
+    const long long *ydims, const long long *stride, const long long *pad,
+    const long long *dil);
+AFHIP_KAPI int afhip_k_conv2_nn_gemm(int pass, int in_type, int out_type,
+                                     int precision, void *out, const void *in,
+                                     const void *w, const long long *xdims,
+                                     const long long *wdims,
+                                     const long long *ydims,
+                                     const long long *stride,
+                                     const long long *pad,
+                                     const long long *dil, void *workspace,
+                                     long long workspace_bytes, void *stream);
+
 /* Workspace bytes afhip_k_conv2_nn_wgrad uses for its split reduction over
  * N*OH*OW (0 when it runs unsplit). */
 AFHIP_KAPI long long afhip_k_conv2_nn_wgrad_workspace(
@@ -160,6 +182,14 @@ AFHIP_KAPI int afhip_k_conv2_nn_f64(int pass, void *out, const void *a0,
  * uses afhip_k_conv2_nn_fwd/_dgrad. For pass 0 `in` is x and `out` is y; for
  * pass 1 `in` is dy and `out` is dx. */
 AFHIP_KAPI long long afhip_k_conv2_nn_patch_workspace(
+    int pass, int in_type, int precision, const long long *xdims,
+    const long long *wdims, const long long *ydims, const long long *stride,
+    const long long *pad, const long long *dil);
+/* Workgroups the patch path would launch for this geometry (its persistent
+ * kernels: at most the tile count), or 0 when it does not serve it. The
+ * dispatch sends layers with too few tiles to fill the chip to the split
+ * gather GEMM (afhip_k_conv2_nn_gemm). */
+AFHIP_KAPI long long afhip_k_conv2_nn_patch_tiles(
     int pass, int in_type, int precision, const long long *xdims,
     const long long *wdims, const long long *ydims, const long long *stride,
     const long long *pad, const long long *dil);

@@ -279,6 +279,7 @@ def test_separable_arg_errors(af):
 def test_patch_path_matches_gather_path(af, rng, g, prec, monkeypatch):
     """The LDS-patch kernels (forward, stride-1 data gradient) and the gather
     kernels compute the same sums in different orders."""
+    monkeypatch.setenv("AFHIP_NN_SMALL", "0")
     x, w = _data(rng, g)
     y = af.convolve2NN(x, w, g[2], g[3], g[4], precision=prec)
     dy = rng.standard_normal(y.dims).astype(np.float32)

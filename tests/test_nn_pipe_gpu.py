@@ -18,6 +18,13 @@ import oracle
 
 pytestmark = pytest.mark.gpu
 
+
+@pytest.fixture(autouse=True)
+def _patch_kernels_for_small_layers(monkeypatch):
+    """These small layers exercise the patch kernels, not the split GEMM that
+    the dispatch would otherwise pick for them."""
+    monkeypatch.setenv("AFHIP_NN_SMALL", "0")
+
 # (x dims [W,H,C,N], w dims [fw,fh,C,K], stride, pad, dilation)
 PIPE_GEOMS = [
     ((16, 12, 40, 3), (3, 3, 40, 24), (1, 1), (1, 1), (1, 1)),    # 64-row tile, 2 chunks with a tail

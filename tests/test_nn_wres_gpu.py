@@ -20,6 +20,13 @@ import oracle
 
 pytestmark = pytest.mark.gpu
 
+
+@pytest.fixture(autouse=True)
+def _patch_kernels_for_small_layers(monkeypatch):
+    """These small layers exercise the patch kernels, not the split GEMM that
+    the dispatch would otherwise pick for them."""
+    monkeypatch.setenv("AFHIP_NN_SMALL", "0")
+
 # (x dims [W,H,C,N], w dims [fw,fh,C,K], stride, pad, dilation)
 WRES_GEOMS = [
     ((56, 56, 64, 4), (3, 3, 64, 128), (1, 1), (1, 1), (1, 1)),
