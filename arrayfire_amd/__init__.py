@@ -87,7 +87,8 @@ EXPORTED = [
     "afhip_k_iir_workspace", "afhip_k_iir", "afhip_k_conv2_nn_patch_noalloc_limit",
     "afhip_k_conv2_nn_patch_noalloc", "afhip_k_conv2_nn_dgrad_poly_workspace",
     "afhip_k_conv2_nn_dgrad_poly", "afhip_k_conv2_nn_gemm_workspace", "afhip_k_conv2_nn_gemm",
-    "afhip_k_conv2_nn_patch_tiles",
+    "afhip_k_conv2_nn_patch_tiles", "afhip_nn_step_plan_create", "afhip_nn_step_plan_run",
+    "afhip_nn_step_plan_release",
 ]
 
 
@@ -154,6 +155,10 @@ def lib() -> ctypes.CDLL:
         L.afhip_convolve2_gradient_nn_ex.argtypes = [ctypes.POINTER(vp), vp, vp, vp, vp, u, i64p,
                                                      u, i64p, u, i64p, ctypes.c_int, ctypes.c_int,
                                                      ctypes.c_int]
+        L.afhip_nn_step_plan_create.argtypes = ([ctypes.POINTER(vp)] * 4 + [vp, vp, vp] + [u, i64p] * 3
+                                                + [ctypes.c_int, ctypes.c_int])
+        L.afhip_nn_step_plan_run.argtypes = [vp]
+        L.afhip_nn_step_plan_release.argtypes = [vp]
         ll = ctypes.c_longlong
         L.afhip_k_convolve.argtypes = [ctypes.c_int, vp, i64p, vp, i64p, vp, i64p, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_int, vp]
@@ -480,6 +485,32 @@ class RaggedPlan:
     def __del__(self):
         if _lib is not None and getattr(self, "_plan", None) and self._plan.value:
             _lib.afhip_ragged_plan_release(self._plan)
+            self._plan = ctypes.c_void_p()
+
+
+class NNStepPlan:
+    """afhip_nn_step_plan: the convolve2_nn forward and both adjoints of one
+    training step captured as one hipGraph. ``y``, ``dx`` and ``dw`` are
+    rewritten by every run() from the current x, w and dy."""
+
+    def __init__(self, x, w, dy, stride=(1, 1), padding=(0, 0), dilation=(1, 1),
+                 precision: int | None = None, out_type: int | None = None):
+        self._in = [_arr(x), _arr(w), _arr(dy)]
+        st, pd, dl = _two(stride), _two(padding), _two(dilation)
+        self._plan = ctypes.c_void_p()
+        hs = [ctypes.c_void_p() for _ in range(3)]
+        check(lib().afhip_nn_step_plan_create(
+            ctypes.byref(self._plan), *[ctypes.byref(h) for h in hs],
+            *[a.handle for a in self._in], len(stride), st, len(padding), pd, len(dilation), dl,
+            precision or 0, -1 if out_type is None else out_type))
+        self.y, self.dx, self.dw = [Array(handle=h.value) for h in hs]
+
+    def run(self) -> None:
+        check(lib().afhip_nn_step_plan_run(self._plan))
+
+    def __del__(self):
+        if _lib is not None and getattr(self, "_plan", None) and self._plan.value:
+            _lib.afhip_nn_step_plan_release(self._plan)
             self._plan = ctypes.c_void_p()
 
 

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <memory>
 #include <vector>
 
 #include "../../include/af/array.h"
@@ -713,6 +714,113 @@ AFAPI af_err afhip_convolve2_gradient_nn_ex(
     return nn_gradient(out, incoming_gradient, original_signal, original_filter, convolved_output,
                        stride_dims, strides, padding_dims, paddings, dilation_dims, dilations,
                        grad_type, (int)precision, out_type, false);
+}
+
+}  // extern "C"
+
+struct afhip_nn_step_plan_t {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<void *> kept;                            // workspaces the graph uses
+    std::vector<std::shared_ptr<afhip::Buffer>> arrays;  // x, w, dy, y, dx, dw
+    ~afhip_nn_step_plan_t() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (void *p : kept) afhip::pool_free(p);
+    }
+};
+
+extern "C" {
+
+// The three passes of one step through the public entry points (checks,
+// dispatch, allocation); the handles are released on failure.
+static af_err nn_step_passes(af_array *y, af_array *dx, af_array *dw, const af_array x, const af_array w,
+                             const af_array dy, const unsigned sdims, const dim_t *strides, const unsigned pdims,
+                             const dim_t *paddings, const unsigned ddims, const dim_t *dilations, int precision,
+                             af_dtype out_type) {
+    *y = *dx = *dw = nullptr;
+    af_err e = nn_forward(y, x, w, sdims, strides, pdims, paddings, ddims, dilations, precision, out_type);
+    if (e == AF_SUCCESS)
+        e = nn_gradient(dx, dy, x, w, *y, sdims, strides, pdims, paddings, ddims, dilations,
+                        AF_CONV_GRADIENT_DATA, precision, out_type, false);
+    if (e == AF_SUCCESS)
+        e = nn_gradient(dw, dy, x, w, *y, sdims, strides, pdims, paddings, ddims, dilations,
+                        AF_CONV_GRADIENT_FILTER, precision, out_type, false);
+    if (e != AF_SUCCESS) {
+        for (af_array *h : {y, dx, dw})
+            if (*h) {
+                destroy_handle(*h);
+                *h = nullptr;
+            }
+    }
+    return e;
+}
+
+AFAPI af_err afhip_nn_step_plan_create(afhip_nn_step_plan *plan, af_array *y, af_array *dx, af_array *dw,
+                                       const af_array x, const af_array w, const af_array dy,
+                                       const unsigned stride_dims, const dim_t *strides,
+                                       const unsigned padding_dims, const dim_t *paddings,
+                                       const unsigned dilation_dims, const dim_t *dilations,
+                                       afhip_precision precision, af_dtype out_type) {
+    try {
+        ARG_ASSERT(0, plan != nullptr && y != nullptr && dx != nullptr && dw != nullptr);
+        // eager run: argument checks, first-use kernel setup, pool warm-up
+        af_array e[3];
+        af_err rc = nn_step_passes(&e[0], &e[1], &e[2], x, w, dy, stride_dims, strides, padding_dims, paddings,
+                                   dilation_dims, dilations, (int)precision, out_type);
+        if (rc != AF_SUCCESS) return rc;
+        for (af_array h : e) destroy_handle(h);
+        sync_stream();
+
+        auto p = std::make_unique<afhip_nn_step_plan_t>();
+        af_array h[3] = {nullptr, nullptr, nullptr};
+        hipStream_t s = stream();
+        check_hip(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+        pool_keep(&p->kept);
+        rc = nn_step_passes(&h[0], &h[1], &h[2], x, w, dy, stride_dims, strides, padding_dims, paddings,
+                            dilation_dims, dilations, (int)precision, out_type);
+        pool_keep(nullptr);
+        const hipError_t ce = hipStreamEndCapture(s, &p->graph);
+        if (rc != AF_SUCCESS || ce != hipSuccess) {
+            for (af_array a : h)
+                if (a) destroy_handle(a);
+            if (rc != AF_SUCCESS) return rc;
+            check_hip(ce, "hipStreamEndCapture");
+        }
+        const hipError_t ie = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            for (af_array a : h) destroy_handle(a);
+            check_hip(ie, "hipGraphInstantiate");
+        }
+        for (af_array a : {x, w, dy}) p->arrays.push_back(get(a).buf);
+        for (af_array a : h) p->arrays.push_back(get(a).buf);
+        *y = h[0];
+        *dx = h[1];
+        *dw = h[2];
+        *plan = p.release();
+    }
+    CATCHALL;
+    return AF_SUCCESS;
+}
+
+AFAPI af_err afhip_nn_step_plan_run(afhip_nn_step_plan plan) {
+    try {
+        ARG_ASSERT(0, plan != nullptr);
+        check_hip(hipGraphLaunch(plan->exec, stream()), "hipGraphLaunch");
+    }
+    CATCHALL;
+    return AF_SUCCESS;
+}
+
+AFAPI af_err afhip_nn_step_plan_release(afhip_nn_step_plan plan) {
+    try {
+        if (plan) {
+            sync_stream();  // a replay in flight still uses the workspaces
+            delete plan;
+        }
+    }
+    CATCHALL;
+    return AF_SUCCESS;
 }
 
 // include/af/data.h:515

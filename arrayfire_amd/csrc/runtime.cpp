@@ -156,8 +156,18 @@ void *pool_alloc(size_t bytes) {
     return p;
 }
 
+namespace {
+thread_local std::vector<void *> *t_keep = nullptr;
+}
+
+void pool_keep(std::vector<void *> *keep) { t_keep = keep; }
+
 void pool_free(void *p) {
     if (!p) return;
+    if (t_keep) {
+        t_keep->push_back(p);
+        return;
+    }
     Pool &P = pool();
     std::lock_guard<std::mutex> g(P.m);
     auto it = P.sizes.find(p);

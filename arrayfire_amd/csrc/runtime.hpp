@@ -13,6 +13,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../../include/af/defines.h"
 #include "../../include/af/hip.h"
@@ -101,6 +102,11 @@ void ensure_device();
 void *pool_alloc(size_t bytes);
 void pool_free(void *p);
 void pool_gc();
+// While a graph is captured on the library stream, buffers returned to the
+// pool by this thread are deferred into *keep instead (the graph still uses
+// them); pool_keep(nullptr) ends that. The caller returns them later with
+// pool_free.
+void pool_keep(std::vector<void *> *keep);
 void pool_info(size_t *alloc_bytes, size_t *alloc_buffers, size_t *lock_bytes,
                size_t *lock_buffers);
 

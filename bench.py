@@ -37,7 +37,7 @@ PEAK = {"hbm_GBs": 8000.0, "bf16_TFs": 2500.0, "fp32_TFs": 157.3,
         "fp32_valu_TFs": 78.6}
 # Mangled names of the step's kernels (as rocprofv3 reports them).
 KERNEL_SYMBOL = {
-    "conv_fwd": "afhip::conv_pipe<1, unsigned short, unsigned short, 2, 3, 0, 3>",
+    "conv_fwd": "afhip::conv_wres<1, unsigned short, unsigned short, 2, 4, 0>",
     "conv_dgrad": "afhip::conv_pipe<1, unsigned short, unsigned short, 1, 3, 1, 3>",
     "wgrad_partial": "afhip::wgrad_pc<3, 3, 1, 0, true>",
     "wgrad_reduce": "afhip::wgrad_sum<unsigned short>",
@@ -215,6 +215,23 @@ def time_af_api(af, ev, stream, X, W, DY, out_type, steps, warmup=2, precision=N
     ev.record(e1, stream)
     ms = ev.ms(e0, e1) / steps
     del hold
+    return ms
+
+
+def time_graph(af, ev, stream, X, W, DY, steps, warmup):
+    """K replays of the step captured by afhip_nn_step_plan between one event
+    pair on the library stream (ms per step)."""
+    plan = af.NNStepPlan(X, W, DY, ST, PD, DL)
+    for _ in range(warmup):
+        plan.run()
+    af.sync()
+    e0, e1 = ev.create(2)
+    ev.record(e0, stream)
+    for _ in range(steps):
+        plan.run()
+    ev.record(e1, stream)
+    ms = ev.ms(e0, e1) / steps
+    del plan
     return ms
 
 
@@ -675,6 +692,8 @@ def main():
     wall_bf16 = time.perf_counter() - t0
     ms_f32 = time_af_api(af, ev, stream, X32, W32, DY32, af.f32, max(3, args.steps // 2),
                          max(1, args.warmup // 2))
+    # the same BF16 step as one captured hipGraph (afhip_nn_step_plan)
+    ms_graph = time_graph(af, ev, stream, Xb, Wb, DYb, args.steps, args.warmup)
     # per-kernel durations: the same step through the kernel ABI with an event
     # after every launch (the events add small gaps; the af_* timing above has
     # none)
@@ -733,6 +752,8 @@ def main():
                                "56x56), w [3,3,64,128], stride 1, pad 1, dilation 1; bf16 "
                                "operands and results, fp32 accumulation",
                    "global_batch": XD[3], "parallelism": "none (1 GPU)"},
+        "ms_per_step_hipgraph": round(ms_graph, 4),
+        "tflops_hipgraph": round(3 * flops_pass / (ms_graph * 1e-3) / 1e12, 3),
         "fp32_plain_tflops": round(tf_f32, 3),
         "bf16_over_fp32": round(tf_bf16 / tf_f32, 3),
         "ms_per_pass_bf16": {n: round(float(v), 4) for n, v in zip(names, per_bf16)},

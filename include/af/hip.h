@@ -72,6 +72,30 @@ AFAPI af_err afhip_convolve2_gradient_nn_ex(
     af_conv_gradient_type grad_type, afhip_precision precision,
     af_dtype out_type);
 
+/* A convolve2_nn training step as one captured hipGraph: the forward
+ * af_convolve2_nn(y, x, w, ...) and both adjoints
+ * af_convolve2_gradient_nn(dx | dw, dy, x, w, y, ..., DATA | FILTER), with
+ * the arithmetic of afhip_convolve2_nn_ex / afhip_convolve2_gradient_nn_ex
+ * (precision, out_type). The plan runs the three passes once eagerly (the
+ * same argument checks and error codes as those calls), then captures them
+ * on the library stream with the kernels, tiles and workspaces the dispatch
+ * picks, and allocates y, dx and dw (new handles, owned by the caller; the
+ * plan keeps their storage and that of x, w and dy alive until it is
+ * released). afhip_nn_step_plan_run replays the graph: it reads the current
+ * contents of x, w and dy (weight packing included) and rewrites y, dx and
+ * dw. No counterpart in the reference (its CUDA backend launches each pass
+ * separately, src/backend/cuda/convolveNN.cpp). */
+typedef struct afhip_nn_step_plan_t *afhip_nn_step_plan;
+AFAPI af_err afhip_nn_step_plan_create(
+    afhip_nn_step_plan *plan, af_array *y, af_array *dx, af_array *dw,
+    const af_array x, const af_array w, const af_array dy,
+    const unsigned stride_dims, const dim_t *strides,
+    const unsigned padding_dims, const dim_t *paddings,
+    const unsigned dilation_dims, const dim_t *dilations,
+    afhip_precision precision, af_dtype out_type);
+AFAPI af_err afhip_nn_step_plan_run(afhip_nn_step_plan plan);
+AFAPI af_err afhip_nn_step_plan_release(afhip_nn_step_plan plan);
+
 /* Ragged batches: count independent 2-D spatial convolutions, each exactly
  * af_convolve2(&outputs[i], signals[i], filters[i], modes[i],
  * AF_CONV_SPATIAL) for an f32 signal with exactly 2 dims (dim1 > 1) and a
