@@ -182,7 +182,7 @@ def lib() -> ctypes.CDLL:
         L.afhip_k_conv2_nn_dgrad_poly_workspace.argtypes = [ctypes.c_int] * 3 + [i64p] * 6
         L.afhip_k_conv2_nn_dgrad_poly_workspace.restype = ll
         L.afhip_k_conv2_nn_dgrad_poly.argtypes = [ctypes.c_int] * 3 + [vp, vp, vp] + [i64p] * 6 + [vp, ll, vp]
-        L.afhip_k_conv2_nn_gemm_workspace.argtypes = [ctypes.c_int] + [i64p] * 6
+        L.afhip_k_conv2_nn_gemm_workspace.argtypes = [ctypes.c_int] * 3 + [i64p] * 6
         L.afhip_k_conv2_nn_gemm_workspace.restype = ll
         L.afhip_k_conv2_nn_gemm.argtypes = [ctypes.c_int] * 4 + [vp, vp, vp] + [i64p] * 6 + [vp, ll, vp]
         L.afhip_k_conv2_nn_patch_tiles.argtypes = [ctypes.c_int] * 3 + [i64p] * 6

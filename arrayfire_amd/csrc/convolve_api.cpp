@@ -220,7 +220,7 @@ static bool run_small(int pass, const ArrayImpl &in, const ArrayImpl &w, af_dtyp
     const long long tiles = afhip_k_conv2_nn_patch_tiles(pass, in.type, precision, g.x.d, g.w.d, g.y.d, g.st,
                                                          g.pd, g.dl);
     if (tiles > 0 && tiles * 4 >= device_cus()) return false;
-    const long long need = afhip_k_conv2_nn_gemm_workspace(pass, g.x.d, g.w.d, g.y.d, g.st, g.pd, g.dl);
+    const long long need = afhip_k_conv2_nn_gemm_workspace(pass, in.type, precision, g.x.d, g.w.d, g.y.d, g.st, g.pd, g.dl);
     if (need <= 0) return false;
     auto ws = scratch((size_t)need);
     check_k(afhip_k_conv2_nn_gemm(pass, in.type, out_type, precision, out, in.data(), w.data(), g.x.d, g.w.d,
@@ -248,7 +248,7 @@ static bool run_patch(int pass, const ArrayImpl &in, const ArrayImpl &w, af_dtyp
 // the chip (workspace from the pool).
 static void run_gemm(int pass, const ArrayImpl &in, const ArrayImpl &w, af_dtype out_type, int precision,
                      void *out, const NNGeom &g) {
-    const long long need = afhip_k_conv2_nn_gemm_workspace(pass, g.x.d, g.w.d, g.y.d, g.st, g.pd, g.dl);
+    const long long need = afhip_k_conv2_nn_gemm_workspace(pass, in.type, precision, g.x.d, g.w.d, g.y.d, g.st, g.pd, g.dl);
     std::shared_ptr<Buffer> ws;
     if (need > 0) ws = scratch((size_t)need);
     check_k(afhip_k_conv2_nn_gemm(pass, in.type, out_type, precision, out, in.data(), w.data(), g.x.d, g.w.d,

@@ -30,7 +30,9 @@
 //   SPLIT hi = bf16(v), lo = bf16(v - hi); hi*hi' + hi*lo' + lo*hi' on BF16
 // Accumulation is FP32 in every mode. One LDS buffer; the next slice's global
 // loads are in flight in registers while the current slice is multiplied.
+#include <algorithm>
 #include <cstdlib>
+#include <type_traits>
 
 #include "kcommon.hpp"
 #include "../../../include/afhip_kernels.h"
@@ -69,6 +71,12 @@ struct NNParams {
     int phased;
     int nph;
     int ph_tile0[kMaxPhases + 1];
+    // Channel-last operands (forward and data gradient, operands stored in
+    // the MMA type): packed weights Wp [RS][M][Rcp] and the activation (x,
+    // or dy for the data gradient) as [N][rows][cols][Rcp], Rcp = the
+    // reduction channels rounded up to the 32-deep slice, zero past them.
+    int cl;
+    int Rcp;
 };
 
 // Phase extents: pixels of residue r along an axis of length L and stride s,
@@ -441,8 +449,146 @@ struct Gather<TIn, P_WGRAD> {
     }
 };
 
+// ---- channel-last gathers: each lane's 16 R values are one contiguous run
+// of 16 channels (2 or 4 aligned 16-byte loads) for A and for B.
+template <typename T, int PASS>
+struct GatherCL {
+    static constexpr int Q = (int)sizeof(T);  // 16-byte vectors per 16 values
+    const T *act, *wp;
+    const NNParams &p;
+    long long arow;
+    bool bvalid;
+    int bn, ba, bb;   // image; output row, column (forward) or input y, x (data gradient)
+    int rx, ry, Ti;   // phased data gradient: the tile's phase, its tap count along W
+    __device__ GatherCL(const T *act_, const T *wp_, const NNParams &p_, long long ar, long long br, int ph)
+        : act(act_), wp(wp_), p(p_), arow(ar) {
+        rx = ry = 0;
+        Ti = p.fw;
+        if (PASS == P_DGRAD && p.phased) {
+            rx = ph % p.s0;
+            ry = ph / p.s0;
+            const int Mx = phase_len(p.W, p.s0, rx), My = phase_len(p.H, p.s1, ry);
+            bvalid = br < (long long)p.N * My * Mx;
+            const Pix q = decode(bvalid ? br : 0, My, Mx);
+            bn = q.n;
+            ba = p.s1 * q.a + ry;
+            bb = p.s0 * q.b + rx;
+            Ti = phase_taps(p.fw, rx, p.p0, p.d0, p.s0);
+        } else {
+            bvalid = br < p.Ncol;
+            const Pix q = PASS == P_FWD ? decode(bvalid ? br : 0, p.OH, p.OW) : decode(bvalid ? br : 0, p.H, p.W);
+            bn = q.n;
+            ba = q.a;
+            bb = q.b;
+        }
+    }
+    __device__ void load(long long ks, int kk0, uint4 (&ra)[Q], uint4 (&rb)[Q]) const {
+        const int u = (int)(ks / p.kchunks);
+        const int c0 = (int)(ks - (long long)u * p.kchunks) * BK + kk0;
+        int i, j;
+        if (PASS == P_DGRAD && p.phased) {
+            i = phase_tap_at(p.fw, rx, p.p0, p.d0, p.s0, u % Ti);
+            j = phase_tap_at(p.fh, ry, p.p1, p.d1, p.s1, u / Ti);
+        } else {
+            i = u % p.fw;
+            j = u / p.fw;
+        }
+        const int t = j * p.fw + i;
+        const uint4 z = make_uint4(0, 0, 0, 0);
+        if (arow < p.M) {
+            const uint4 *a = reinterpret_cast<const uint4 *>(wp + ((long long)t * p.M + arow) * p.Rcp + c0);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) ra[q] = a[q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) ra[q] = z;
+        }
+        bool ok;
+        long long off;
+        if constexpr (PASS == P_FWD) {
+            const int ih = ba * p.s1 - p.p1 + j * p.d1;
+            const int iw = bb * p.s0 - p.p0 + i * p.d0;
+            ok = bvalid && ih >= 0 && ih < p.H && iw >= 0 && iw < p.W;
+            off = (((long long)bn * p.H + ih) * p.W + iw) * p.Rcp + c0;
+        } else {
+            const int nh = ba + p.p1 - j * p.d1;
+            const int nw = bb + p.p0 - i * p.d0;
+            ok = bvalid && nh >= 0 && nw >= 0 && (nh % p.s1) == 0 && (nw % p.s0) == 0;
+            const int oh = ok ? nh / p.s1 : 0, ow = ok ? nw / p.s0 : 0;
+            ok = ok && oh < p.OH && ow < p.OW;
+            off = (((long long)bn * p.OH + oh) * p.OW + ow) * p.Rcp + c0;
+        }
+        if (ok) {
+            const uint4 *b = reinterpret_cast<const uint4 *>(act + off);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) rb[q] = b[q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) rb[q] = z;
+        }
+    }
+};
+
+// Writes one lane's 16 raw values (row, R offset kk0) of an operand tile.
+template <int MODE>
+__device__ inline void store_raw16(char *tile, int row, int kk0, const uint4 *v) {
+    constexpr int P = Lds<MODE>::PITCH, E = Lds<MODE>::ESZ;
+    uint4 *t = reinterpret_cast<uint4 *>(tile + (row * P + kk0) * E);
+#pragma unroll
+    for (int q = 0; q < E; ++q) t[q] = v[q];
+}
+
+// Wp [RS][M][Rcp]: forward M = K, R = C, Wp[t][k][c] = w[k][c][fh-1-j][fw-1-i];
+// data gradient M = C, R = K, Wp[t][c][k] = the same element (t = j*fw + i).
+template <typename T>
+__global__ __launch_bounds__(256) void nn_cl_pack(T *__restrict__ wp, const T *__restrict__ w, NNParams p,
+                                                  int dgrad) {
+    const int Rc = dgrad ? p.K : p.C;
+    const long long total = (long long)p.RS * p.M * p.Rcp;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int r = (int)(e % p.Rcp);
+        const long long mt = e / p.Rcp;
+        const int m = (int)(mt % p.M);
+        const int t = (int)(mt / p.M);
+        const int i = t % p.fw, j = t / p.fw;
+        T v{};
+        if (r < Rc) {
+            const int k = dgrad ? r : m, c = dgrad ? m : r;
+            v = w[(((long long)k * p.C + c) * p.fh + (p.fh - 1 - j)) * p.fw + (p.fw - 1 - i)];
+        }
+        wp[e] = v;
+    }
+}
+
+// dst [N][rows][cols][Cp] <- src [N][Cd][rows][cols] (NCHW to channel-last),
+// zero for the channels past Cd. One lane writes 16 bytes of channels.
+template <typename T>
+__global__ __launch_bounds__(256) void nn_cl_transpose(T *__restrict__ dst, const T *__restrict__ src, int cols,
+                                                       int rows, int Cd, int N, int Cp) {
+    constexpr int G = 16 / (int)sizeof(T);
+    const int cg = Cp / G;
+    const long long hw = (long long)rows * cols;
+    const long long total = (long long)N * hw * cg;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int g = (int)(e % cg);
+        const long long pix = e / cg;  // n * hw + y * cols + x
+        const long long n = pix / hw, yx = pix - n * hw;
+        T v[G];
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+            const int c = g * G + q;
+            v[q] = c < Cd ? src[(n * Cd + c) * hw + yx] : T{};
+        }
+        uint4 o;
+        __builtin_memcpy(&o, v, 16);
+        *reinterpret_cast<uint4 *>(dst + pix * Cp + g * G) = o;
+    }
+}
+
 // ------------------------------------------------------------- kernel
-template <int MODE, typename TIn, typename TOut, int PASS>
+template <int MODE, typename TIn, typename TOut, int PASS, bool CL = false>
 __global__ __launch_bounds__(NT) void nn_gemm(TOut *__restrict__ out, const TIn *__restrict__ in0,
                                               const TIn *__restrict__ in1, float *__restrict__ partial,
                                               NNParams p) {
@@ -474,8 +620,6 @@ __global__ __launch_bounds__(NT) void nn_gemm(TOut *__restrict__ out, const TIn 
     const int wm = wave >> 1, wn = wave & 1;
     const int lrow = tid & 127, kk0 = (tid >> 7) * 16;
 
-    Gather<TIn, PASS> g(in0, in1, p, m0 + lrow, n0 + lrow, ph);
-
     f32x4_t acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -486,15 +630,32 @@ __global__ __launch_bounds__(NT) void nn_gemm(TOut *__restrict__ out, const TIn 
     long long ks1 = ks0 + kps;
     if (ks1 > ksteps) ks1 = ksteps;
 
-    float va[16], vb[16];
-    if (ks0 < ks1) g.load(ks0, kk0, va, vb);
-    for (long long ks = ks0; ks < ks1; ++ks) {
-        __syncthreads();
-        store_row16<MODE>(tA, lrow, kk0, va);
-        store_row16<MODE>(tB, lrow, kk0, vb);
-        __syncthreads();
-        if (ks + 1 < ks1) g.load(ks + 1, kk0, va, vb);
-        mma_slice<MODE>(tA, tB, wm, wn, lane, acc);
+    if constexpr (CL) {
+        // in0 = the channel-last activation, in1 = the packed weights
+        GatherCL<TIn, PASS> g(in0, in1, p, m0 + lrow, n0 + lrow, ph);
+        constexpr int Q = GatherCL<TIn, PASS>::Q;
+        uint4 ra[Q], rb[Q];
+        if (ks0 < ks1) g.load(ks0, kk0, ra, rb);
+        for (long long ks = ks0; ks < ks1; ++ks) {
+            __syncthreads();
+            store_raw16<MODE>(tA, lrow, kk0, ra);
+            store_raw16<MODE>(tB, lrow, kk0, rb);
+            __syncthreads();
+            if (ks + 1 < ks1) g.load(ks + 1, kk0, ra, rb);
+            mma_slice<MODE>(tA, tB, wm, wn, lane, acc);
+        }
+    } else {
+        Gather<TIn, PASS> g(in0, in1, p, m0 + lrow, n0 + lrow, ph);
+        float va[16], vb[16];
+        if (ks0 < ks1) g.load(ks0, kk0, va, vb);
+        for (long long ks = ks0; ks < ks1; ++ks) {
+            __syncthreads();
+            store_row16<MODE>(tA, lrow, kk0, va);
+            store_row16<MODE>(tB, lrow, kk0, vb);
+            __syncthreads();
+            if (ks + 1 < ks1) g.load(ks + 1, kk0, va, vb);
+            mma_slice<MODE>(tA, tB, wm, wn, lane, acc);
+        }
     }
 
     // Epilogue. acc[i][j][v] is C[row][col] with
@@ -643,6 +804,8 @@ static int setup(NNParams &p, int pass, const long long *xd, const long long *wd
     p.nsplit = 1;
     p.phased = 0;
     p.nph = 1;
+    p.cl = 0;
+    p.Rcp = 0;
     for (int i = 0; i <= kMaxPhases; ++i) p.ph_tile0[i] = 0;
     if (pass == P_FWD) {
         p.M = p.K;
@@ -699,6 +862,39 @@ static int wgrad_splits(const NNParams &p) {
     return (int)want;
 }
 
+// The channel-last path runs when the operands are stored in the MMA's
+// operand type (FP32, BF16 or FP16 arrays at their own precision).
+template <int MODE, typename TIn>
+constexpr bool cl_eligible() {
+    return (MODE == M_F32 && std::is_same<TIn, float>::value) ||
+           (MODE == M_BF16 && std::is_same<TIn, bf16s>::value) ||
+           (MODE == M_F16 && std::is_same<TIn, _Float16>::value);
+}
+
+// Workspace of the forward / data-gradient GEMM: packed weights and the
+// channel-last activation (esz > 0: element bytes of the channel-last path),
+// then the split partials; every part 256-byte aligned.
+// AFHIP_NN_CL=0 in the environment disables the channel-last path.
+struct GemmLayout {
+    int cl = 0, Rcp = 0;
+    long long off_w = 0, off_act = 0, off_part = 0, total = 0;
+};
+static GemmLayout gemm_layout(const NNParams &p, int pass, int esz, int nsplit) {
+    GemmLayout L;
+    auto al = [](long long b) { return (b + 255) & ~255LL; };
+    const char *env = std::getenv("AFHIP_NN_CL");
+    if (esz > 0 && !(env && env[0] == '0')) {
+        L.cl = 1;
+        L.Rcp = p.kchunks * BK;
+        const long long rows = pass == P_FWD ? p.H : p.OH, cols = pass == P_FWD ? p.W : p.OW;
+        L.off_w = 0;
+        L.off_act = al((long long)p.RS * p.M * L.Rcp * esz);
+        L.off_part = L.off_act + al((long long)p.N * rows * cols * L.Rcp * esz);
+    }
+    L.total = L.off_part + (nsplit > 1 ? (long long)nsplit * p.M * p.Ncol * 4 : 0);
+    return L;
+}
+
 // Split count for the forward and data gradient: layers whose output tiles
 // leave most CUs idle split the reduction (FP32 partial tiles, a second
 // summing pass), about two workgroups per CU, each with at least 4 slices,
@@ -731,16 +927,46 @@ static int run(int pass, void *out, const void *a0, const void *a1, NNParams &p,
     const long long tiles = p.tilesM * p.tilesN;
     if (pass == P_FWD || pass == P_DGRAD) {
         p.nsplit = gemm_splits(p);
-        if (p.nsplit > 1 && (ws == nullptr || ws_bytes < (long long)p.nsplit * p.M * p.Ncol * 4)) p.nsplit = 1;
+        constexpr bool kCL = cl_eligible<MODE, TIn>();
+        GemmLayout L = gemm_layout(p, pass, kCL ? (int)sizeof(TIn) : 0, p.nsplit);
+        if (ws == nullptr || ws_bytes < L.total) {
+            // a short workspace: drop the split, then the channel-last copies
+            p.nsplit = 1;
+            L = gemm_layout(p, pass, kCL ? (int)sizeof(TIn) : 0, 1);
+            if (ws == nullptr || ws_bytes < L.total) L = gemm_layout(p, pass, 0, 1);
+        }
         p.ksteps_per_split = (p.ksteps + p.nsplit - 1) / p.nsplit;
-        auto fn = pass == P_FWD ? nn_gemm<MODE, TIn, TOut, P_FWD> : nn_gemm<MODE, TIn, TOut, P_DGRAD>;
-        fn<<<dim3((unsigned)(tiles * p.nsplit)), NT, smem, s>>>((TOut *)out, (const TIn *)a0, (const TIn *)a1,
-                                                               (float *)ws, p);
+        char *wsb = (char *)ws;
+        float *part = p.nsplit > 1 ? (float *)(wsb + L.off_part) : nullptr;
+        const unsigned grid = (unsigned)(tiles * p.nsplit);
+        if constexpr (kCL) {
+            if (L.cl) {
+                p.cl = 1;
+                p.Rcp = L.Rcp;
+                TIn *wp = (TIn *)(wsb + L.off_w), *act = (TIn *)(wsb + L.off_act);
+                const long long nw = (long long)p.RS * p.M * p.Rcp;
+                long long b1 = (nw + 255) / 256;
+                nn_cl_pack<TIn><<<(unsigned)std::min<long long>(b1, kMaxGridBlocks), 256, 0, s>>>(
+                    wp, (const TIn *)a1, p, pass == P_DGRAD);
+                const int rows = pass == P_FWD ? p.H : p.OH, cols = pass == P_FWD ? p.W : p.OW;
+                const int Cd = pass == P_FWD ? p.C : p.K;
+                const long long na = (long long)p.N * rows * cols * (p.Rcp / (16 / (int)sizeof(TIn)));
+                long long b2 = (na + 255) / 256;
+                nn_cl_transpose<TIn><<<(unsigned)std::min<long long>(b2, kMaxGridBlocks), 256, 0, s>>>(
+                    act, (const TIn *)a0, cols, rows, Cd, p.N, p.Rcp);
+                auto fn = pass == P_FWD ? nn_gemm<MODE, TIn, TOut, P_FWD, true> : nn_gemm<MODE, TIn, TOut, P_DGRAD, true>;
+                fn<<<grid, NT, smem, s>>>((TOut *)out, act, wp, part, p);
+            }
+        }
+        if (!p.cl) {
+            auto fn = pass == P_FWD ? nn_gemm<MODE, TIn, TOut, P_FWD> : nn_gemm<MODE, TIn, TOut, P_DGRAD>;
+            fn<<<grid, NT, smem, s>>>((TOut *)out, (const TIn *)a0, (const TIn *)a1, part, p);
+        }
         if (p.nsplit > 1) {
             long long blocks = (p.M * p.Ncol + 255) / 256;
             if (blocks > kMaxGridBlocks) blocks = kMaxGridBlocks;
             auto red = pass == P_FWD ? nn_split_reduce<TOut, P_FWD> : nn_split_reduce<TOut, P_DGRAD>;
-            red<<<dim3((unsigned)blocks), 256, 0, s>>>((TOut *)out, (const float *)ws, p);
+            red<<<dim3((unsigned)blocks), 256, 0, s>>>((TOut *)out, part, p);
         }
     } else {
         if (p.ksteps == 0) {
@@ -806,15 +1032,20 @@ static int dispatch(int pass, int in_type, int out_type, int precision, void *ou
 
 using namespace afhip;
 
-extern "C" long long afhip_k_conv2_nn_gemm_workspace(int pass, const long long *xdims, const long long *wdims,
-                                                     const long long *ydims, const long long *stride,
-                                                     const long long *pad, const long long *dil) {
+extern "C" long long afhip_k_conv2_nn_gemm_workspace(int pass, int in_type, int precision, const long long *xdims,
+                                                     const long long *wdims, const long long *ydims,
+                                                     const long long *stride, const long long *pad,
+                                                     const long long *dil) {
     if (pass != P_FWD && pass != P_DGRAD) return 0;
     NNParams p;
     if (setup(p, pass, xdims, wdims, ydims, stride, pad, dil)) return 0;
     if (p.M == 0 || p.Ncol == 0) return 0;
-    const int ns = gemm_splits(p);
-    return ns > 1 ? (long long)ns * p.M * p.Ncol * 4 : 0;
+    // channel-last path: operands stored at their own precision
+    const bool cl = (in_type == DT_F32 && (precision == 0 || precision == 4)) ||
+                    (in_type == DT_BF16 && (precision == 0 || precision == 1)) ||
+                    (in_type == DT_F16 && (precision == 0 || precision == 2));
+    const int esz = cl ? (in_type == DT_F32 ? 4 : 2) : 0;
+    return gemm_layout(p, pass, esz, gemm_splits(p)).total;
 }
 
 extern "C" int afhip_k_conv2_nn_gemm(int pass, int in_type, int out_type, int precision, void *out,

@@ -19,6 +19,7 @@
 // into blocks of R_oh output rows of one image, staged in LDS; the blocks are
 // split across workgroups, whose FP32 partial tiles a second kernel sums in a
 // fixed order (deterministic).
+#include <algorithm>
 #include <cstdlib>
 #include <type_traits>
 
@@ -49,6 +50,7 @@ struct WGParams {
     int bufb;            // bytes per block buffer (16-byte multiple)
     int dy_lpr_log2;     // producer copy: log2 of lanes per dy row (>= segrow)
     int x_rpr;           // producer copy: x rows per round (256 / (PWpad / 8))
+    int esz;             // operand bytes: 2 (BF16 / FP16) or 4 (FP32, wgrad_f32)
 };
 
 __device__ inline uint4 pack8(const uint16_t *h) {
@@ -471,6 +473,166 @@ __global__ __launch_bounds__(2 * WG_THREADS, 2) void wgrad_pc(float *__restrict_
             for (int v = 0; v < 4; ++v) dst[((mi * RS + t) * 4 + v) * 64] = acc[mi][t][v];
 }
 
+// ---------------------------------------------------------------- FP32
+// The FP32 filter gradient on v_mfma_f32_16x16x4f32 with the same blocking
+// (64 filters x 64 channels x all taps per workgroup, row blocks of output
+// pixels staged in LDS, FP32 split partials in the lane-contiguous layout
+// that wgrad_sum adds). One reduction value per lane per MFMA: in step s,
+// lane group g = lane / 16 owns the 8-pixel segment 4s + g and MFMA q of a
+// tap takes its pixel q. A lane reads its 8 dy values per filter row tile
+// as two 16-byte LDS reads and, per filter row, one window of 7*S0 + FW x
+// values that serves all FW taps. The FP32 MFMA (32 cycles per SIMD) bounds
+// the kernel; one workgroup per CU stages a block while the next waits.
+template <bool ALIGNED>
+__device__ inline void load_f32(const float *__restrict__ x, const float *__restrict__ dy, float *lds_dy,
+                                float *lds_x, const WGParams &p, int n, int oh0, int kt, int ct) {
+    constexpr int NB = 8;
+    const int tid = threadIdx.x;
+    const long long HW = (long long)p.H * p.W, OHW = (long long)p.OH * p.OW;
+    // dy: WG_BK rows of steps*4 segments of 8 pixels, as 4-value chunks
+    const int cpr = p.steps * 8;
+    const int dchunks = WG_BK * cpr;
+    for (int e0 = 0; e0 < dchunks; e0 += NB * WG_THREADS) {
+        float4 v[NB];
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+            const int e = e0 + q * WG_THREADS + tid;
+            const int m = e / cpr, cj = e - (e / cpr) * cpr;
+            const int seg = cj >> 1;
+            const int k = kt * WG_BK + m;
+            const int oh = oh0 + seg / p.segrow;
+            const int ow0 = (seg % p.segrow) * 8 + (cj & 1) * 4;
+            const bool ok = e < dchunks && seg < p.nseg && k < p.K && oh < p.OH;
+            const float *src = dy + ((long long)n * p.K + k) * OHW + (long long)oh * p.OW + ow0;
+            if constexpr (ALIGNED) {
+                v[q] = (ok && ow0 < p.OW) ? *reinterpret_cast<const float4 *>(src) : make_float4(0.f, 0.f, 0.f, 0.f);
+            } else {
+                float h[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) h[t] = (ok && ow0 + t < p.OW) ? src[t] : 0.f;
+                v[q] = make_float4(h[0], h[1], h[2], h[3]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+            const int e = e0 + q * WG_THREADS + tid;
+            if (e < dchunks) {
+                const int m = e / cpr, cj = e - (e / cpr) * cpr;
+                *reinterpret_cast<float4 *>(lds_dy + m * p.dyp + cj * 4) = v[q];
+            }
+        }
+    }
+    // x: WG_BC channels x PR rows x PWpad/4 chunks; LDS column q holds image
+    // column q - PADL
+    const int qrow = p.PWpad / 4;
+    const int xchunks = WG_BC * p.PR * qrow;
+    for (int e0 = 0; e0 < xchunks; e0 += NB * WG_THREADS) {
+        float4 v[NB];
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+            const int e = e0 + q * WG_THREADS + tid;
+            const int cr = e / qrow, qq = e - (e / qrow) * qrow;
+            const int c = cr / p.PR, r = cr - (cr / p.PR) * p.PR;
+            const int ch = ct * WG_BC + c;
+            const int ih = oh0 * p.s1 - p.p1 + r;
+            const int iw0 = qq * 4 - p.PADL;
+            const bool ok = e < xchunks && ch < p.C && ih >= 0 && ih < p.H;
+            const float *src = x + ((long long)n * p.C + ch) * HW + (long long)ih * p.W;
+            if constexpr (ALIGNED) {
+                v[q] = (ok && iw0 >= 0 && iw0 < p.W) ? *reinterpret_cast<const float4 *>(src + iw0)
+                                                     : make_float4(0.f, 0.f, 0.f, 0.f);
+            } else {
+                float h[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int iw = iw0 + t;
+                    h[t] = (ok && iw >= 0 && iw < p.W) ? src[iw] : 0.f;
+                }
+                v[q] = make_float4(h[0], h[1], h[2], h[3]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+            const int e = e0 + q * WG_THREADS + tid;
+            if (e < xchunks) *reinterpret_cast<float4 *>(lds_x + e * 4) = v[q];
+        }
+    }
+}
+
+template <int FW, int FH, int S0, bool ALIGNED>
+__global__ __launch_bounds__(WG_THREADS, 2) void wgrad_f32(float *__restrict__ out, const float *__restrict__ x,
+                                                           const float *__restrict__ dy, WGParams p) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    float *lds_dy = reinterpret_cast<float *>(smem);
+    float *lds_x = lds_dy + WG_BK * p.dyp;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    long long bid = blockIdx.x;
+    const int split = (int)(bid % p.nsplit);
+    bid /= p.nsplit;
+    const int ct = (int)(bid % p.ctiles);
+    const int kt = (int)(bid / p.ctiles);
+
+    constexpr int RS = FW * FH;
+    constexpr int NWIN = 7 * S0 + FW;
+    wf32x4 acc[4][RS];
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+        for (int t = 0; t < RS; ++t) acc[mi][t] = wf32x4{0.f, 0.f, 0.f, 0.f};
+
+    const int g = lane >> 4, r16 = lane & 15;
+    const int cl = wave * 16 + r16;  // lane's channel within the tile
+    const int u0 = split * p.upsplit;
+    const int u1 = min(u0 + p.upsplit, p.total_units);
+    for (int u = u0; u < u1; ++u) {
+        const int n = u / p.units_per_img;
+        __syncthreads();  // the previous block is no longer read
+        load_f32<ALIGNED>(x, dy, lds_dy, lds_x, p, n, (u - n * p.units_per_img) * p.R_oh, kt, ct);
+        __syncthreads();
+        for (int s = 0; s < p.steps; ++s) {
+            int seg = s * 4 + g;
+            float a[4][8];
+#pragma unroll
+            for (int mi = 0; mi < 4; ++mi) {
+                const float4 *ap = reinterpret_cast<const float4 *>(lds_dy + (mi * 16 + r16) * p.dyp + seg * 8);
+                const float4 lo = ap[0], hi = ap[1];
+                a[mi][0] = lo.x; a[mi][1] = lo.y; a[mi][2] = lo.z; a[mi][3] = lo.w;
+                a[mi][4] = hi.x; a[mi][5] = hi.y; a[mi][6] = hi.z; a[mi][7] = hi.w;
+            }
+            if (seg >= p.nseg) seg = p.nseg - 1;  // dummy segment: A is zero there
+            const int ro = seg / p.segrow;
+            const int ow0 = (seg - ro * p.segrow) * 8;
+            const int e0 = ow0 * S0 + (p.PADL - p.p0);
+#pragma unroll
+            for (int j = 0; j < FH; ++j) {
+                const float *rowp = lds_x + (cl * p.PR + ro * p.s1 + j * p.d1) * p.PWpad + e0;
+                float b[NWIN];
+#pragma unroll
+                for (int q = 0; q < NWIN; ++q) b[q] = rowp[q];
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+#pragma unroll
+                    for (int i = 0; i < FW; ++i)
+#pragma unroll
+                        for (int mi = 0; mi < 4; ++mi)
+                            acc[mi][j * FW + i] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                                a[mi][q], b[q * S0 + i], acc[mi][j * FW + i], 0, 0, 0);
+            }
+        }
+    }
+
+    // same partial layout as wgrad_rows
+    float *dst = out + ((long long)split * p.ktiles * p.ctiles + kt * p.ctiles + ct) * (4096LL * RS) +
+                 (long long)wave * (1024 * RS) + lane;
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+        for (int t = 0; t < RS; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) dst[((mi * RS + t) * 4 + v) * 64] = acc[mi][t][v];
+}
+
 // Sums nsplit partial dW buffers and converts to the output type. A split's
 // partials are tiles (kt, ct) of [wave][mi][tap][v][lane] floats (4096 * RS
 // per tile); every partial element is read coalesced and written to the dW
@@ -548,7 +710,7 @@ __global__ __launch_bounds__(64 * WS_GROUPS) void wgrad_sum(TOut *__restrict__ o
 }
 
 static size_t wg_lds(const WGParams &p) {
-    return (size_t)WG_BK * p.dyp * 2 + (size_t)WG_BC * p.PR * p.PWpad * 2;
+    return ((size_t)WG_BK * p.dyp + (size_t)WG_BC * p.PR * p.PWpad) * (size_t)p.esz;
 }
 
 // Plans the blocking; returns false when the geometry is outside this kernel.
@@ -579,9 +741,17 @@ static bool wg_plan(WGParams &p, int fw, int s0, int d0) {
     p.x_rpr = WG_THREADS / (p.PWpad / 8);
     // AFHIP_NN_WGRAD_PC=0 in the environment selects the single-buffer kernel
     const char *pc_env = std::getenv("AFHIP_NN_WGRAD_PC");
-    const bool pc_ok = !(pc_env && pc_env[0] == '0') && p.dy_lpr_log2 <= 8 && p.x_rpr >= 1 &&
+    const bool pc_ok = !(pc_env && pc_env[0] == '0') && p.esz == 2 && p.dy_lpr_log2 <= 8 && p.x_rpr >= 1 &&
                        (long long)p.C * p.H * p.W * 2 < (1LL << 31) - 64 &&
                        (long long)p.K * p.OH * p.OW * 2 < (1LL << 31) - 64;
+    // AFHIP_NN_WGRAD_F32_LDS=<bytes> in the environment sets another LDS
+    // budget for the FP32 kernel (up to WG_LDS_LIMIT; 81920 or less lets two
+    // workgroups share a CU)
+    size_t lds_limit = WG_LDS_LIMIT;
+    if (p.esz == 4) {
+        const char *le = std::getenv("AFHIP_NN_WGRAD_F32_LDS");
+        if (le && le[0] >= '1' && le[0] <= '9') lds_limit = std::min<size_t>(WG_LDS_LIMIT, std::atol(le));
+    }
     int r = 0;
     for (int pc = pc_ok ? 1 : 0; pc >= 0 && r < 1; --pc) {
         for (r = r0; r >= 1; --r) {
@@ -592,7 +762,7 @@ static bool wg_plan(WGParams &p, int fw, int s0, int d0) {
             p.PR = (r - 1) * p.s1 + (p.fh - 1) * p.d1 + 1;
             p.pc = pc;
             p.bufb = (int)((wg_lds(p) + 15) & ~(size_t)15);
-            if (pc ? 2 * (size_t)p.bufb <= WG_PC_LDS : wg_lds(p) <= WG_LDS_LIMIT) break;
+            if (pc ? 2 * (size_t)p.bufb <= WG_PC_LDS : wg_lds(p) <= lds_limit) break;
         }
     }
     if (r < 1) return false;
@@ -629,6 +799,7 @@ static void wg_fill(WGParams &p, const long long *xd, const long long *wd, const
     p.p0 = (int)pd[0];
     p.p1 = (int)pd[1];
     p.d1 = (int)dl[1];
+    p.esz = 2;
 }
 
 template <int FW, int FH, int S0, int F16>
@@ -662,6 +833,22 @@ static int wg_launch(float *part, const void *x, const void *dy, const WGParams 
     return launch_status();
 }
 
+template <int FW, int FH, int S0>
+static int wf_launch(float *part, const void *x, const void *dy, const WGParams &p, hipStream_t s) {
+    const bool aligned = (p.W & 3) == 0 && (p.OW & 3) == 0;
+    auto fn = aligned ? wgrad_f32<FW, FH, S0, true> : wgrad_f32<FW, FH, S0, false>;
+    static bool attr[2] = {false, false};
+    if (!attr[aligned]) {
+        if (hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)WG_LDS_LIMIT) != hipSuccess)
+            return K_ERR_RUNTIME;
+        attr[aligned] = true;
+    }
+    const long long grid = (long long)p.ktiles * p.ctiles * p.nsplit;
+    fn<<<dim3((unsigned)grid), WG_THREADS, wg_lds(p), s>>>(part, (const float *)x, (const float *)dy, p);
+    return launch_status();
+}
+
 }  // namespace afhip
 
 using namespace afhip;
@@ -672,9 +859,14 @@ extern "C" long long afhip_k_conv2_nn_wgrad_rows_workspace(int in_type, int prec
                                                           const long long *pd, const long long *dl) {
     const bool bf = (in_type == DT_BF16 && (precision == 0 || precision == 1));
     const bool hf = (in_type == DT_F16 && (precision == 0 || precision == 2));
-    if (!bf && !hf) return 0;
+    const bool ff = (in_type == DT_F32 && (precision == 0 || precision == 4));
+    if (!bf && !hf && !ff) return 0;
+    // AFHIP_NN_WGRAD_F32=0 in the environment keeps FP32 on the gather kernel
+    const char *env = std::getenv("AFHIP_NN_WGRAD_F32");
+    if (ff && env && env[0] == '0') return 0;
     WGParams p;
     wg_fill(p, xd, wd, yd, st, pd, dl);
+    p.esz = ff ? 4 : 2;
     if (!wg_plan(p, (int)wd[0], (int)st[0], (int)dl[0])) return 0;
     return (long long)p.nsplit * p.ktiles * p.ctiles * 4096 * p.RS * 4;
 }
@@ -690,11 +882,21 @@ static int rows_call(int what, int in_type, int out_type, int precision, void *d
     WGParams p;
     wg_fill(p, xd, wd, yd, st, pd, dl);
     const int fw = (int)wd[0], s0 = (int)st[0];
+    p.esz = in_type == DT_F32 ? 4 : 2;
     wg_plan(p, fw, s0, (int)dl[0]);
     p.is_f16 = in_type == DT_F16;
     hipStream_t s = (hipStream_t)stream;
     float *part = (float *)workspace;
-    if (what & 1) {
+    if ((what & 1) && p.esz == 4) {
+        int rc;
+#define WFS(FWV, FHV) rc = s0 == 1 ? wf_launch<FWV, FHV, 1>(part, x, dy, p, s) : wf_launch<FWV, FHV, 2>(part, x, dy, p, s)
+        if (fw == 3 && p.fh == 3) { WFS(3, 3); }
+        else if (fw == 3) { WFS(3, 1); }
+        else if (p.fh == 3) { WFS(1, 3); }
+        else { WFS(1, 1); }
+#undef WFS
+        if (rc) return rc;
+    } else if (what & 1) {
         int rc;
 #define WGO(FWV, FHV, S0V)                                                                    \
     rc = p.is_f16 ? wg_launch<FWV, FHV, S0V, 1>(part, x, dy, p, s)                           \
@@ -714,6 +916,8 @@ static int rows_call(int what, int in_type, int out_type, int precision, void *d
         const long long total = (long long)p.ktiles * p.ctiles * 4096 * p.RS;
         long long blocks = total / 256;
         if (blocks > kMaxGridBlocks) blocks = kMaxGridBlocks;
+        if (in_type == DT_F32 && out_type != DT_F32)
+            return K_ERR_TYPE;
         if (out_type == DT_F32)
             wgrad_sum<float><<<(unsigned)blocks, 64 * WS_GROUPS, 0, s>>>((float *)dw, part, p);
         else if (out_type == DT_BF16 && in_type == DT_BF16)

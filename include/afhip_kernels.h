@@ -122,16 +122,19 @@ AFHIP_KAPI int afhip_k_conv2_nn_dgrad(int in_type, int out_type, int precision,
 
 /* The gather GEMM behind afhip_k_conv2_nn_fwd (pass 0: in = x, out = y) and
  * afhip_k_conv2_nn_dgrad (pass 1: in = dy, out = dx), for any geometry, with
- * a workspace: layers whose output tiles leave most CUs idle split the
- * reduction into FP32 partial tiles that a second pass sums in a fixed order.
- * Strided data gradients visit only the taps of each stride phase. The query
- * returns the workspace bytes of the split (0: the call runs unsplit, and a
- * NULL or short workspace also runs unsplit). Same dims and codes as
- * afhip_k_conv2_nn_fwd/_dgrad (src/backend/cpu/convolve.cpp:123-211). */
+ * a workspace: operands stored at their own precision are first copied
+ * channel-last (weights packed per tap, the activation NCHW -> NHWC) so that
+ * every gather is a run of 16-byte loads; layers whose output tiles leave
+ * most CUs idle split the reduction into FP32 partial tiles that a second
+ * pass sums in a fixed order. Strided data gradients visit only the taps of
+ * each stride phase. The query returns the workspace bytes (0: none needed);
+ * a NULL or short workspace runs unsplit and without the channel-last
+ * copies. Same dims and codes as afhip_k_conv2_nn_fwd/_dgrad
+ * (src/backend/cpu/convolve.cpp:123-211). */
 AFHIP_KAPI long long afhip_k_conv2_nn_gemm_workspace(
-    int pass, const long long *xdims, const long long *wdims,
-    const long long *ydims, const long long *stride, const long long *pad,
-    const long long *dil);
+    int pass, int in_type, int precision, const long long *xdims,
+    const long long *wdims, const long long *ydims, const long long *stride,
+    const long long *pad, const long long *dil);
 AFHIP_KAPI int afhip_k_conv2_nn_gemm(int pass, int in_type, int out_type,
                                      int precision, void *out, const void *in,
                                      const void *w, const long long *xdims,

@@ -114,3 +114,25 @@ def test_stride2_layer_full_size(af, prec, bound):
     assert oracle.rel_l2(y.to_numpy(), oracle.conv2_nn_direct_f64(x, w, st, pd, dl)) <= bound
     assert oracle.rel_l2(dx.to_numpy(), oracle.conv2_data_grad_direct_f64(dy, x, w, st, pd, dl)) <= bound
     assert oracle.rel_l2(dw.to_numpy(), oracle.conv2_filter_grad_direct_f64(dy, x, w, st, pd, dl)) <= bound
+
+
+@pytest.mark.parametrize("dt", ["f32", "bf16", "f16"])
+@pytest.mark.parametrize("g", GEOMS, ids=_id)
+def test_channel_last_copies_match_direct_gather(af, rng, g, dt, monkeypatch):
+    """The channel-last path (packed weights, NHWC activation) loads the
+    same operand values into the same tiles as the element gather, so both
+    give identical bits; channels past a 32-deep slice are zero-filled."""
+    x, w = _data(rng, g)
+    st, pd, dl = g[2], g[3], g[4]
+    t = {"bf16": af.bf16, "f16": af.f16, "f32": af.f32}[dt]
+    X, W = af.Array(x).astype(t), af.Array(w).astype(t)
+    y0 = af.convolve2NN(X, W, st, pd, dl)
+    DY = af.Array(rng.standard_normal(y0.dims).astype(np.float32)).astype(t)
+    monkeypatch.setenv("AFHIP_NN_GATHER", "1")
+    y = af.convolve2NN(X, W, st, pd, dl).to_numpy()
+    dx = af.convolve2GradientNN(DY, X, W, y0, st, pd, dl, af.AF_CONV_GRADIENT_DATA).to_numpy()
+    monkeypatch.setenv("AFHIP_NN_CL", "0")
+    y1 = af.convolve2NN(X, W, st, pd, dl).to_numpy()
+    dx1 = af.convolve2GradientNN(DY, X, W, y0, st, pd, dl, af.AF_CONV_GRADIENT_DATA).to_numpy()
+    assert np.array_equal(y, y1)
+    assert np.array_equal(dx, dx1)
