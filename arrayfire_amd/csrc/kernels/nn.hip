@@ -77,6 +77,7 @@ struct NNParams {
     // reduction channels rounded up to the 32-deep slice, zero past them.
     int cl;
     int Rcp;
+    int wp_bytes, act_bytes;  // sizes of the channel-last copies (each below 2^31 - 64)
 };
 
 // Phase extents: pixels of residue r along an axis of length L and stride s,
@@ -449,6 +450,9 @@ struct Gather<TIn, P_WGRAD> {
     }
 };
 
+// Slices of the channel-last path in flight per lane (register sets).
+constexpr int CL_DEPTH = 4;
+
 // ---- channel-last gathers: each lane's 16 R values are one contiguous run
 // of 16 channels (2 or 4 aligned 16-byte loads) for A and for B.
 template <typename T, int PASS>
@@ -482,7 +486,11 @@ struct GatherCL {
             bb = q.b;
         }
     }
+    // Buffer loads: an operand outside the image (or a row past M) reads
+    // zeros from an offset past the resource's end, so every lane issues the
+    // same loads and the prefetch ring's waits count them exactly.
     __device__ void load(long long ks, int kk0, uint4 (&ra)[Q], uint4 (&rb)[Q]) const {
+        constexpr int kOOB = 0x7fffffc0;
         const int u = (int)(ks / p.kchunks);
         const int c0 = (int)(ks - (long long)u * p.kchunks) * BK + kk0;
         int i, j;
@@ -494,15 +502,9 @@ struct GatherCL {
             j = u / p.fw;
         }
         const int t = j * p.fw + i;
-        const uint4 z = make_uint4(0, 0, 0, 0);
-        if (arow < p.M) {
-            const uint4 *a = reinterpret_cast<const uint4 *>(wp + ((long long)t * p.M + arow) * p.Rcp + c0);
-#pragma unroll
-            for (int q = 0; q < Q; ++q) ra[q] = a[q];
-        } else {
-#pragma unroll
-            for (int q = 0; q < Q; ++q) ra[q] = z;
-        }
+        const auto rw = __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(wp), (short)0, p.wp_bytes, 0x00020000);
+        const auto ra_ = __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(act), (short)0, p.act_bytes, 0x00020000);
+        const int offa = arow < p.M ? (int)((((long long)t * p.M + arow) * p.Rcp + c0) * (long long)sizeof(T)) : kOOB;
         bool ok;
         long long off;
         if constexpr (PASS == P_FWD) {
@@ -518,13 +520,11 @@ struct GatherCL {
             ok = ok && oh < p.OH && ow < p.OW;
             off = (((long long)bn * p.OH + oh) * p.OW + ow) * p.Rcp + c0;
         }
-        if (ok) {
-            const uint4 *b = reinterpret_cast<const uint4 *>(act + off);
+        const int offb = ok ? (int)(off * (long long)sizeof(T)) : kOOB;
 #pragma unroll
-            for (int q = 0; q < Q; ++q) rb[q] = b[q];
-        } else {
-#pragma unroll
-            for (int q = 0; q < Q; ++q) rb[q] = z;
+        for (int q = 0; q < Q; ++q) {
+            ra[q] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rw, offa + 16 * q, 0, 0));
+            rb[q] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(ra_, offb + 16 * q, 0, 0));
         }
     }
 };
@@ -540,38 +540,41 @@ __device__ inline void store_raw16(char *tile, int row, int kk0, const uint4 *v)
 
 // Wp [RS][M][Rcp]: forward M = K, R = C, Wp[t][k][c] = w[k][c][fh-1-j][fw-1-i];
 // data gradient M = C, R = K, Wp[t][c][k] = the same element (t = j*fw + i).
-template <typename T>
-__global__ __launch_bounds__(256) void nn_cl_pack(T *__restrict__ wp, const T *__restrict__ w, NNParams p,
-                                                  int dgrad) {
-    const int Rc = dgrad ? p.K : p.C;
-    const long long total = (long long)p.RS * p.M * p.Rcp;
-    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
-         e += (long long)gridDim.x * blockDim.x) {
-        const int r = (int)(e % p.Rcp);
-        const long long mt = e / p.Rcp;
-        const int m = (int)(mt % p.M);
-        const int t = (int)(mt / p.M);
-        const int i = t % p.fw, j = t / p.fw;
-        T v{};
-        if (r < Rc) {
-            const int k = dgrad ? r : m, c = dgrad ? m : r;
-            v = w[(((long long)k * p.C + c) * p.fh + (p.fh - 1 - j)) * p.fw + (p.fw - 1 - i)];
-        }
-        wp[e] = v;
-    }
-}
-
+// Both channel-last copies in one launch: the first nbw workgroups pack the
+// weights into Wp (above), the others transpose the activation:
 // dst [N][rows][cols][Cp] <- src [N][Cd][rows][cols] (NCHW to channel-last),
-// zero for the channels past Cd. One lane writes 16 bytes of channels.
+// zero for the channels past Cd, one lane writing 16 bytes of channels.
 template <typename T>
-__global__ __launch_bounds__(256) void nn_cl_transpose(T *__restrict__ dst, const T *__restrict__ src, int cols,
-                                                       int rows, int Cd, int N, int Cp) {
+__global__ __launch_bounds__(256) void nn_cl_copies(T *__restrict__ wp, const T *__restrict__ w, NNParams p,
+                                                    int dgrad, int nbw, T *__restrict__ dst,
+                                                    const T *__restrict__ src, int cols, int rows, int Cd) {
+    if ((int)blockIdx.x < nbw) {
+        const int Rc = dgrad ? p.K : p.C;
+        const long long total = (long long)p.RS * p.M * p.Rcp;
+        for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+             e += (long long)nbw * blockDim.x) {
+            const int r = (int)(e % p.Rcp);
+            const long long mt = e / p.Rcp;
+            const int m = (int)(mt % p.M);
+            const int t = (int)(mt / p.M);
+            const int i = t % p.fw, j = t / p.fw;
+            T v{};
+            if (r < Rc) {
+                const int k = dgrad ? r : m, c = dgrad ? m : r;
+                v = w[(((long long)k * p.C + c) * p.fh + (p.fh - 1 - j)) * p.fw + (p.fw - 1 - i)];
+            }
+            wp[e] = v;
+        }
+        return;
+    }
     constexpr int G = 16 / (int)sizeof(T);
+    const int Cp = p.Rcp, N = p.N;
     const int cg = Cp / G;
     const long long hw = (long long)rows * cols;
     const long long total = (long long)N * hw * cg;
-    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
-         e += (long long)gridDim.x * blockDim.x) {
+    const long long nbt = (long long)gridDim.x - nbw;
+    for (long long e = (blockIdx.x - nbw) * (long long)blockDim.x + threadIdx.x; e < total;
+         e += nbt * blockDim.x) {
         const int g = (int)(e % cg);
         const long long pix = e / cg;  // n * hw + y * cols + x
         const long long n = pix / hw, yx = pix - n * hw;
@@ -631,18 +634,33 @@ __global__ __launch_bounds__(NT) void nn_gemm(TOut *__restrict__ out, const TIn 
     if (ks1 > ksteps) ks1 = ksteps;
 
     if constexpr (CL) {
-        // in0 = the channel-last activation, in1 = the packed weights
+        // in0 = the channel-last activation, in1 = the packed weights. A ring
+        // of CL_DEPTH register sets keeps the loads of the next CL_DEPTH - 1
+        // slices in flight while a slice is stored and multiplied.
+        // Every lane issues the same loads in every slot (slices past the
+        // range reload the last one), so the waits before each store count
+        // exactly the CL_DEPTH - 1 slices still in flight.
         GatherCL<TIn, PASS> g(in0, in1, p, m0 + lrow, n0 + lrow, ph);
         constexpr int Q = GatherCL<TIn, PASS>::Q;
-        uint4 ra[Q], rb[Q];
-        if (ks0 < ks1) g.load(ks0, kk0, ra, rb);
-        for (long long ks = ks0; ks < ks1; ++ks) {
-            __syncthreads();
-            store_raw16<MODE>(tA, lrow, kk0, ra);
-            store_raw16<MODE>(tB, lrow, kk0, rb);
-            __syncthreads();
-            if (ks + 1 < ks1) g.load(ks + 1, kk0, ra, rb);
-            mma_slice<MODE>(tA, tB, wm, wn, lane, acc);
+        if (ks0 < ks1) {
+            uint4 ra[CL_DEPTH][Q], rb[CL_DEPTH][Q];
+            const long long last = ks1 - 1;
+#pragma unroll
+            for (int d = 0; d < CL_DEPTH; ++d) g.load(min(ks0 + d, last), kk0, ra[d], rb[d]);
+            for (long long ks = ks0; ks < ks1; ks += CL_DEPTH) {
+#pragma unroll
+                for (int d = 0; d < CL_DEPTH; ++d) {
+                    const bool live = ks + d < ks1;
+                    __syncthreads();
+                    if (live) {
+                        store_raw16<MODE>(tA, lrow, kk0, ra[d]);
+                        store_raw16<MODE>(tB, lrow, kk0, rb[d]);
+                    }
+                    __syncthreads();
+                    g.load(min(ks + d + CL_DEPTH, last), kk0, ra[d], rb[d]);
+                    if (live) mma_slice<MODE>(tA, tB, wm, wn, lane, acc);
+                }
+            }
         }
     } else {
         Gather<TIn, PASS> g(in0, in1, p, m0 + lrow, n0 + lrow, ph);
@@ -806,6 +824,7 @@ static int setup(NNParams &p, int pass, const long long *xd, const long long *wd
     p.nph = 1;
     p.cl = 0;
     p.Rcp = 0;
+    p.wp_bytes = p.act_bytes = 0;
     for (int i = 0; i <= kMaxPhases; ++i) p.ph_tile0[i] = 0;
     if (pass == P_FWD) {
         p.M = p.K;
@@ -876,20 +895,25 @@ constexpr bool cl_eligible() {
 // then the split partials; every part 256-byte aligned.
 // AFHIP_NN_CL=0 in the environment disables the channel-last path.
 struct GemmLayout {
-    int cl = 0, Rcp = 0;
+    int cl = 0, Rcp = 0, wp_bytes = 0, act_bytes = 0;
     long long off_w = 0, off_act = 0, off_part = 0, total = 0;
 };
 static GemmLayout gemm_layout(const NNParams &p, int pass, int esz, int nsplit) {
     GemmLayout L;
     auto al = [](long long b) { return (b + 255) & ~255LL; };
     const char *env = std::getenv("AFHIP_NN_CL");
-    if (esz > 0 && !(env && env[0] == '0')) {
+    const long long Rcp = (long long)p.kchunks * BK;
+    const long long rows = pass == P_FWD ? p.H : p.OH, cols = pass == P_FWD ? p.W : p.OW;
+    const long long wb = (long long)p.RS * p.M * Rcp * esz, ab = (long long)p.N * rows * cols * Rcp * esz;
+    // the copies are read through buffer resources (32-bit byte offsets)
+    if (esz > 0 && !(env && env[0] == '0') && wb < (1LL << 31) - 64 && ab < (1LL << 31) - 64) {
         L.cl = 1;
-        L.Rcp = p.kchunks * BK;
-        const long long rows = pass == P_FWD ? p.H : p.OH, cols = pass == P_FWD ? p.W : p.OW;
+        L.Rcp = (int)Rcp;
+        L.wp_bytes = (int)wb;
+        L.act_bytes = (int)ab;
         L.off_w = 0;
-        L.off_act = al((long long)p.RS * p.M * L.Rcp * esz);
-        L.off_part = L.off_act + al((long long)p.N * rows * cols * L.Rcp * esz);
+        L.off_act = al(wb);
+        L.off_part = L.off_act + al(ab);
     }
     L.total = L.off_part + (nsplit > 1 ? (long long)nsplit * p.M * p.Ncol * 4 : 0);
     return L;
@@ -943,17 +967,18 @@ static int run(int pass, void *out, const void *a0, const void *a1, NNParams &p,
             if (L.cl) {
                 p.cl = 1;
                 p.Rcp = L.Rcp;
+                p.wp_bytes = L.wp_bytes;
+                p.act_bytes = L.act_bytes;
                 TIn *wp = (TIn *)(wsb + L.off_w), *act = (TIn *)(wsb + L.off_act);
                 const long long nw = (long long)p.RS * p.M * p.Rcp;
-                long long b1 = (nw + 255) / 256;
-                nn_cl_pack<TIn><<<(unsigned)std::min<long long>(b1, kMaxGridBlocks), 256, 0, s>>>(
-                    wp, (const TIn *)a1, p, pass == P_DGRAD);
+                const long long b1 = std::min<long long>((nw + 255) / 256, kMaxGridBlocks / 2);
                 const int rows = pass == P_FWD ? p.H : p.OH, cols = pass == P_FWD ? p.W : p.OW;
                 const int Cd = pass == P_FWD ? p.C : p.K;
                 const long long na = (long long)p.N * rows * cols * (p.Rcp / (16 / (int)sizeof(TIn)));
-                long long b2 = (na + 255) / 256;
-                nn_cl_transpose<TIn><<<(unsigned)std::min<long long>(b2, kMaxGridBlocks), 256, 0, s>>>(
-                    act, (const TIn *)a0, cols, rows, Cd, p.N, p.Rcp);
+                const long long b2 = std::min<long long>((na + 255) / 256, kMaxGridBlocks / 2);
+                nn_cl_copies<TIn><<<(unsigned)(b1 + b2), 256, 0, s>>>(wp, (const TIn *)a1, p, pass == P_DGRAD,
+                                                                      (int)b1, act, (const TIn *)a0, cols, rows,
+                                                                      Cd);
                 auto fn = pass == P_FWD ? nn_gemm<MODE, TIn, TOut, P_FWD, true> : nn_gemm<MODE, TIn, TOut, P_DGRAD, true>;
                 fn<<<grid, NT, smem, s>>>((TOut *)out, act, wp, part, p);
             }
