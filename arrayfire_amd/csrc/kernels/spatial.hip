@@ -9,7 +9,11 @@
 //  * conv2_strip<FX,FY,OFF>: FP32 rank-2 kernel for odd filter sides 3..9 on
 //    rows of whole 16-byte vectors; walks 128-column strips down the image
 //    with an input-row ring in LDS and register prefetch (see the kernel).
-//  * conv3_tile<3,3,3>: FP32 rank-3 kernel, 32x8x8 output blocks.
+//  * conv3_tile<F,F,F>: FP32 rank-3 kernel for 3x3x3 and 5x5x5 filters,
+//    32x8x8 output blocks.
+//  * conv1_tile: FP32 rank-1 kernel for filters of up to 2048 taps, every
+//    batch kind: 1024 outputs per 256-lane workgroup from a window staged in
+//    LDS, 4 outputs per lane, taps read as LDS broadcasts.
 //  * conv2_tile<FX,FY>: FP32 rank-2 kernel. A 128x64 output tile per
 //    256-thread workgroup; the (128+FX-1)x(64+FY-1) input halo tile is staged
 //    in LDS with zero borders so the tap loops are branch-free; each lane keeps
@@ -689,7 +693,98 @@ static int launch_tile3(float *out, const float *in, const float *filt, const Sp
     t.sz = p.expand ? 0 : (int)(p.fd[2] / 2);
     if (t.nblk == 0) return K_OK;
     if (t.nblk > 0x7fffffffLL) return K_ERR_SIZE;
-    conv3_tile<3, 3, 3><<<dim3((unsigned)t.nblk), dim3(256), 0, s>>>(out, in, filt, t);
+    if (p.fd[0] == 5)
+        conv3_tile<5, 5, 5><<<dim3((unsigned)t.nblk), dim3(256), 0, s>>>(out, in, filt, t);
+    else
+        conv3_tile<3, 3, 3><<<dim3((unsigned)t.nblk), dim3(256), 0, s>>>(out, in, filt, t);
+    return launch_status();
+}
+
+// ---------------------------------------------------------------- FP32 1-D
+// y[o] = sum_w x[o + st - w] * f[w] for one slice: the workgroup's 1024
+// outputs need x[o0 + st - (F-1) .. o0 + st + 1023], staged in LDS with zeros
+// outside the signal; lane l computes outputs 4l..4l+3 with w ascending,
+// sliding a 4-value register window one LDS read per tap. Each product is
+// rounded before it is added, in the reference's order, so the result is the
+// reference's bit for bit (af_iir feeds it to a recurrence that amplifies any
+// rounding difference).
+constexpr int T1_OUT = 1024;
+constexpr int T1_MAXF = 2048;
+
+struct Tile1Params {
+    dim_t S0, O0, F;                    // signal, output and filter lengths of one slice
+    dim_t in_step[4], out_step[4], filt_step[4], batch[4];
+    dim_t ntiles, nblk;                 // tiles per slice, workgroups
+    int st;                             // centring offset (0 for EXPAND)
+};
+
+__global__ __launch_bounds__(256) void conv1_tile(float *__restrict__ out, const float *__restrict__ in,
+                                                  const float *__restrict__ filt, Tile1Params p) {
+    __shared__ float xs[T1_OUT + T1_MAXF];
+    __shared__ float wt[T1_MAXF];
+    const dim_t blk = blockIdx.x;
+    const dim_t tile = blk % p.ntiles;
+    dim_t sl = blk / p.ntiles;
+    const dim_t b1 = sl % p.batch[1];
+    sl /= p.batch[1];
+    const dim_t b2 = sl % p.batch[2], b3 = sl / p.batch[2];
+    in += b1 * p.in_step[1] + b2 * p.in_step[2] + b3 * p.in_step[3];
+    out += b1 * p.out_step[1] + b2 * p.out_step[2] + b3 * p.out_step[3];
+    filt += b1 * p.filt_step[1] + b2 * p.filt_step[2] + b3 * p.filt_step[3];
+    const int F = (int)p.F;
+    const dim_t o0 = tile * T1_OUT;
+    const dim_t base = o0 + p.st - (F - 1);
+    const int nx = T1_OUT + F - 1;
+    for (int k = threadIdx.x; k < nx; k += 256) {
+        const dim_t g = base + k;
+        xs[k] = (g >= 0 && g < p.S0) ? in[g] : 0.f;
+    }
+    for (int k = threadIdx.x; k < F; k += 256) wt[k] = filt[k];
+    __syncthreads();
+    const int l4 = 4 * (int)threadIdx.x;
+    // window v[k] = xs[l4 + k + F - 1 - w] for the current tap w
+    float v0 = xs[l4 + F - 1], v1 = xs[l4 + F], v2 = xs[l4 + F + 1], v3 = xs[l4 + F + 2];
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+    for (int w = 0; w < F; ++w) {
+#pragma clang fp contract(off)
+        const float f = wt[w];
+        a0 = a0 + v0 * f;
+        a1 = a1 + v1 * f;
+        a2 = a2 + v2 * f;
+        a3 = a3 + v3 * f;
+        v3 = v2;
+        v2 = v1;
+        v1 = v0;
+        if (w + 1 < F) v0 = xs[l4 + F - 2 - w];
+    }
+    const dim_t o = o0 + l4;
+    if (o + 3 < p.O0 && ((reinterpret_cast<uintptr_t>(out + o) & 15) == 0)) {
+        *reinterpret_cast<float4 *>(out + o) = make_float4(a0, a1, a2, a3);
+    } else {
+        const float a[4] = {a0, a1, a2, a3};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (o + k < p.O0) out[o + k] = a[k];
+    }
+}
+
+static int launch_tile1(float *out, const float *in, const float *filt, const SpatialParams &p, hipStream_t s) {
+    Tile1Params t;
+    t.S0 = p.sd[0];
+    t.O0 = p.od[0];
+    t.F = p.fd[0];
+    for (int d = 0; d < 4; ++d) {
+        t.in_step[d] = p.in_step[d];
+        t.out_step[d] = p.out_step[d];
+        t.filt_step[d] = p.filt_step[d];
+        t.batch[d] = p.batch[d];
+    }
+    t.st = p.expand ? 0 : (int)(p.fd[0] / 2);
+    t.ntiles = (t.O0 + T1_OUT - 1) / T1_OUT;
+    t.nblk = t.ntiles * p.batch[1] * p.batch[2] * p.batch[3];
+    if (t.nblk == 0) return K_OK;
+    if (t.nblk > 0x7fffffffLL) return K_ERR_SIZE;
+    conv1_tile<<<dim3((unsigned)t.nblk), dim3(256), 0, s>>>(out, in, filt, t);
     return launch_status();
 }
 
@@ -711,9 +806,16 @@ extern "C" int afhip_k_convolve(int dtype, void *out, const long long *odims,
         Tile2Fn fn = tile2_lookup(fdims[0], fdims[1]);
         if (fn || strip) return launch_tile2((float *)out, (const float *)in, (const float *)filt, p, fn, strip, s);
     }
-    if (dtype == DT_F32 && rank == 3 && p.batch[1] == 1 && p.batch[2] == 1 && fdims[0] == 3 && fdims[1] == 3 &&
-        fdims[2] == 3)
+    if (dtype == DT_F32 && rank == 3 && p.batch[1] == 1 && p.batch[2] == 1 && fdims[0] == fdims[1] &&
+        fdims[1] == fdims[2] && (fdims[0] == 3 || fdims[0] == 5))
         return launch_tile3((float *)out, (const float *)in, (const float *)filt, p, s);
+    // AFHIP_CONV1_TILE=0 in the environment keeps FP32 1-D on conv_generic
+    static const bool tile1_off = [] {
+        const char *e = getenv("AFHIP_CONV1_TILE");
+        return e && e[0] == '0';
+    }();
+    if (dtype == DT_F32 && rank == 1 && fdims[0] >= 1 && fdims[0] <= T1_MAXF && !tile1_off)
+        return launch_tile1((float *)out, (const float *)in, (const float *)filt, p, s);
     switch (dtype) {
         case DT_F32: return launch_generic<float, float>(out, in, filt, p, s);
         case DT_F64: return launch_generic<double, double>(out, in, filt, p, s);
