@@ -89,6 +89,13 @@ EXPORTED = [
     "afhip_k_conv2_nn_dgrad_poly", "afhip_k_conv2_nn_gemm_workspace", "afhip_k_conv2_nn_gemm",
     "afhip_k_conv2_nn_patch_tiles", "afhip_nn_step_plan_create", "afhip_nn_step_plan_run",
     "afhip_nn_step_plan_release",
+    # include/af/features.h, include/af/vision.h
+    "af_create_features", "af_retain_features", "af_get_features_num", "af_get_features_xpos",
+    "af_get_features_ypos", "af_get_features_score", "af_get_features_orientation",
+    "af_get_features_size", "af_release_features", "af_harris",
+    "afhip_k_harris_deriv", "afhip_k_harris_nms", "afhip_k_select_flagged_workspace",
+    "afhip_k_select_flagged", "afhip_k_sort_desc_workspace", "afhip_k_harris_sort",
+    "afhip_k_harris_out",
 ]
 
 
@@ -158,6 +165,13 @@ def lib() -> ctypes.CDLL:
         L.afhip_nn_step_plan_create.argtypes = ([ctypes.POINTER(vp)] * 4 + [vp, vp, vp] + [u, i64p] * 3
                                                 + [ctypes.c_int, ctypes.c_int])
         L.afhip_nn_step_plan_run.argtypes = [vp]
+        L.af_harris.argtypes = [ctypes.POINTER(vp), vp, u, ctypes.c_float, ctypes.c_float, u, ctypes.c_float]
+        L.af_create_features.argtypes = [ctypes.POINTER(vp), ctypes.c_longlong]
+        L.af_retain_features.argtypes = [ctypes.POINTER(vp), vp]
+        L.af_get_features_num.argtypes = [ctypes.POINTER(ctypes.c_longlong), vp]
+        for n in ("xpos", "ypos", "score", "orientation", "size"):
+            getattr(L, "af_get_features_" + n).argtypes = [ctypes.POINTER(vp), vp]
+        L.af_release_features.argtypes = [vp]
         L.afhip_nn_step_plan_release.argtypes = [vp]
         ll = ctypes.c_longlong
         L.afhip_k_convolve.argtypes = [ctypes.c_int, vp, i64p, vp, i64p, vp, i64p, ctypes.c_int,
@@ -512,6 +526,48 @@ class NNStepPlan:
         if _lib is not None and getattr(self, "_plan", None) and self._plan.value:
             _lib.afhip_nn_step_plan_release(self._plan)
             self._plan = ctypes.c_void_p()
+
+
+class Features:
+    """af_features: x (dim1 index), y (dim0 index), score, orientation and
+    size vectors of n features. The arrays belong to the features handle."""
+
+    def __init__(self, handle: int):
+        self._h = ctypes.c_void_p(handle)
+
+    @property
+    def n(self) -> int:
+        v = ctypes.c_longlong()
+        check(lib().af_get_features_num(ctypes.byref(v), self._h))
+        return v.value
+
+    def _get(self, name: str) -> np.ndarray:
+        h = ctypes.c_void_p()
+        check(getattr(lib(), "af_get_features_" + name)(ctypes.byref(h), self._h))
+        a = Array(handle=h.value)
+        try:
+            return a.to_numpy().reshape(-1)[: self.n]
+        finally:
+            a._h = ctypes.c_void_p()  # the features handle owns the array
+
+    def arrays(self):
+        """(x, y, score, orientation, size) as numpy vectors."""
+        return tuple(self._get(k) for k in ("xpos", "ypos", "score", "orientation", "size"))
+
+    def __del__(self):
+        if _lib is not None and getattr(self, "_h", None) and self._h.value:
+            _lib.af_release_features(self._h)
+            self._h = ctypes.c_void_p()
+
+
+def harris(image, max_corners: int = 500, min_response: float = 1e5, sigma: float = 1.0,
+           block_size: int = 0, k_thr: float = 0.04) -> Features:
+    """af::harris (include/af/vision.h:402)."""
+    x = _arr(image)
+    h = ctypes.c_void_p()
+    check(lib().af_harris(ctypes.byref(h), x.handle, int(max_corners), float(min_response),
+                          float(sigma), int(block_size), float(k_thr)))
+    return Features(h.value)
 
 
 def convolve2_ragged(signals, filters, modes) -> list:

@@ -373,3 +373,40 @@ AFHIP_KAPI int afhip_k_conv2_nn_dgrad_poly(int in_type, int out_type,
                                            void *workspace,
                                            long long workspace_bytes,
                                            void *stream);
+
+/* Harris corner detector kernels (af_harris, src/api/c/harris.cpp:52-95;
+ * src/backend/cpu/harris.cpp:26-148; src/backend/cpu/kernel/harris.hpp).
+ * dtype f32 or f64; W = dim0, H = dim1 of the image.
+ * deriv: d3 [W,H,3] <- (ix*ix, ix*iy, iy*iy), iy/ix the gradients along
+ * dim0/dim1 (src/backend/cpu/kernel/gradient.hpp:18-80). */
+AFHIP_KAPI int afhip_k_harris_deriv(int dtype, void *d3, const void *in,
+                                    long long W, long long H, void *stream);
+/* nms: flags[i] = 1 where the response det - k*tr^2 of the smoothed s3
+ * products exceeds its 8 neighbours and is >= min_r, for pixels at least
+ * `border` from every edge; resp[i] = the response as float there. */
+AFHIP_KAPI int afhip_k_harris_nms(int dtype, unsigned char *flags,
+                                  float *resp, const void *s3, long long W,
+                                  long long H, float k_thr, float min_r,
+                                  int border, void *stream);
+/* Stable compaction of the flagged indices (ascending) into out_idx, their
+ * count into *d_num (device); the workspace query returns its bytes. */
+AFHIP_KAPI long long afhip_k_select_flagged_workspace(long long n);
+AFHIP_KAPI int afhip_k_select_flagged(const unsigned char *flags, long long n,
+                                      unsigned *out_idx, unsigned *d_num,
+                                      void *temp, long long temp_bytes,
+                                      void *stream);
+/* Stable descending sort of resp[sel[i]], i < m (keys/vals: m-entry
+ * buffers; the sorted keys and positions go to keys_sorted/vals_sorted). */
+AFHIP_KAPI long long afhip_k_sort_desc_workspace(long long m);
+AFHIP_KAPI int afhip_k_harris_sort(const unsigned *sel, const float *resp,
+                                   long long m, float *keys, unsigned *vals,
+                                   float *keys_sorted, unsigned *vals_sorted,
+                                   void *temp, long long temp_bytes,
+                                   void *stream);
+/* The first n features: x = sel[i] / W, y = sel[i] % W, score, orientation
+ * 0, size 1, with i = order[f] (or f when order is NULL). */
+AFHIP_KAPI int afhip_k_harris_out(float *x, float *y, float *score,
+                                  float *orient, float *size,
+                                  const unsigned *sel, const unsigned *order,
+                                  const float *sorted_keys, const float *resp,
+                                  long long W, long long n, void *stream);

@@ -37,6 +37,7 @@ taken in Fortran order, which is ArrayFire's column-major memory order.
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 import subprocess
 
@@ -693,3 +694,114 @@ def dog(x, r1: int, r2: int, windows=None) -> np.ndarray:
         windows = (gaussian_kernel(2 * r1 + 1, 2 * r1 + 1), gaussian_kernel(2 * r2 + 1, 2 * r2 + 1))
     g1, g2 = (np.asarray(w, dtype=np.float32).astype(at) for w in windows)
     return convolve(xa, g1, 2, False) - convolve(xa, g2, 2, False)
+
+
+# ------------------------------------------------------------------ Harris
+def harris_filter(filter_len: int, sigma: float, dtype=np.float32) -> np.ndarray:
+    """The window of cpu::harris (src/backend/cpu/harris.cpp:34-44): a box
+    1/filter_len for sigma < 0.5, else gaussian1D (src/backend/cpu/utility.hpp:35-48)
+    evaluated per tap in double, stored in T, summed and normalised in T."""
+    dt = np.dtype(dtype).type
+    if sigma < 0.5:
+        return np.full(filter_len, dt(1) / dt(filter_len), dtype=dtype)
+    s = sigma if sigma > 0 else 0.25 * filter_len
+    out = np.zeros(filter_len, dtype=dtype)
+    tot = dt(0)
+    for i in range(filter_len):
+        x = i - (filter_len - 1) // 2
+        el = dt(1.0 / math.sqrt(2 * math.pi * s * s) * math.exp(-((x * x) / (2 * (s * s)))))
+        out[i] = el
+        tot = dt(tot + el)
+    for k in range(filter_len):
+        out[k] = dt(out[k] / tot)
+    return out
+
+
+def gradient2(x: np.ndarray):
+    """cpu::kernel::gradient (src/backend/cpu/kernel/gradient.hpp:18-80) for a
+    2-D array: (grad along dim0, grad along dim1), central differences times
+    0.5 inside, one-sided differences at the two ends, in T."""
+    dt = x.dtype.type
+    g0 = np.empty_like(x)
+    g1 = np.empty_like(x)
+    g0[1:-1, :] = dt(0.5) * (x[2:, :] - x[:-2, :])
+    g0[0, :] = x[1, :] - x[0, :]
+    g0[-1, :] = x[-1, :] - x[-2, :]
+    g1[:, 1:-1] = dt(0.5) * (x[:, 2:] - x[:, :-2])
+    g1[:, 0] = x[:, 1] - x[:, 0]
+    g1[:, -1] = x[:, -1] - x[:, -2]
+    return g0, g1
+
+
+def harris(img: np.ndarray, max_corners: int, min_response: float, sigma: float,
+           block_size: int, k_thr: float):
+    """af_harris (src/api/c/harris.cpp:52-95) with cpu::harris
+    (src/backend/cpu/harris.cpp:26-148) and its kernels
+    (src/backend/cpu/kernel/harris.hpp:18-118). Returns (x, y, score) as
+    float32 vectors: x is the dim1 index, y the dim0 index, in detection
+    order (dim1 outer, dim0 inner) or, when max_corners > 0 and more corners
+    were found, the max_corners strongest after a stable descending sort.
+    Raises ValueError(af_err) where the reference's asserts fail."""
+    x = np.asarray(img)
+    if x.ndim != 2:
+        raise ValueError(203)
+    if x.dtype not in (np.float32, np.float64):
+        raise ValueError(204)
+    T = x.dtype.type
+    filter_len = block_size if block_size != 0 else int(math.floor(np.float32(6.0) * np.float32(sigma)))
+    if block_size == 0 and filter_len % 2 == 0:
+        filter_len -= 1
+    edge = block_size // 2 if block_size > 0 else filter_len // 2
+    if not (x.shape[0] >= 2 * edge + 1 or x.shape[1] >= 2 * edge + 1):
+        raise ValueError(202)
+    if not (max_corners > 0 or min_response > 0.0):
+        raise ValueError(202)
+    if not (np.float32(k_thr) >= np.float32(0.01)):
+        raise ValueError(202)
+    if not (block_size > 2 or (sigma >= 0.5 and sigma <= 5.0)):
+        raise ValueError(202)
+    if not (block_size <= 32):
+        raise ValueError(202)
+    h = harris_filter(filter_len, sigma, x.dtype)
+    border_len = filter_len // 2 + 1
+    iy, ix = gradient2(x)
+    ixx = ix * ix
+    ixy = ix * iy
+    iyy = iy * iy
+    ixx = convolve2_sep(h, h, ixx, False)[:, :, 0, 0]
+    ixy = convolve2_sep(h, h, ixy, False)[:, :, 0, 0]
+    iyy = convolve2_sep(h, h, iyy, False)[:, :, 0, 0]
+    d0, d1 = x.shape
+    corner_lim = int(np.float32(x.size) * np.float32(0.2))
+    r = border_len
+    resp = np.zeros_like(x)
+    sl = (slice(r, d0 - r), slice(r, d1 - r))
+    tr = ixx[sl] + iyy[sl]
+    det = ixx[sl] * iyy[sl] - ixy[sl] * ixy[sl]
+    resp[sl] = det - T(np.float32(k_thr)) * (tr * tr)
+    min_r = 0 if max_corners > 0 else int(min_response)
+    mr = T(np.float32(min_r))
+    xs, ys, rs = [], [], []
+    count = 0
+    r1 = border_len + 1
+    for xi in range(r1, d1 - r1):
+        for yi in range(r1, d0 - r1):
+            v = resp[yi, xi]
+            nb = resp[yi - 1:yi + 2, xi - 1:xi + 2].copy()
+            nb[1, 1] = -np.inf
+            if v > nb.max() and v >= mr:
+                if count < corner_lim:
+                    xs.append(np.float32(xi))
+                    ys.append(np.float32(yi))
+                    rs.append(np.float32(v))
+                count += 1
+    xs, ys, rs = (np.array(a, dtype=np.float32) for a in (xs, ys, rs))
+    corners_out = min(count, max_corners if max_corners > 0 else corner_lim)
+    if corners_out == 0:
+        e = np.zeros(0, np.float32)
+        return e, e, e
+    if max_corners > 0 and count > corners_out:
+        order = np.argsort(-rs, kind="stable")
+        order = order[:corners_out]
+        return xs[order], ys[order], rs[order]
+    return xs[:corners_out], ys[:corners_out], rs[:corners_out]

@@ -513,3 +513,71 @@ def test_dog_constant_image_and_scipy():
     g2 = oracle.gaussian_kernel(3, 3).astype(np.float64)
     ref = ss.convolve2d(x, g1, mode="same") - ss.convolve2d(x, g2, mode="same")
     np.testing.assert_allclose(d, ref, rtol=1e-12, atol=1e-12)
+
+
+# ------------------------------------------------------------------ Harris
+def test_harris_square_corners():
+    """A bright rectangle on a dark field: exactly its four corners, with
+    equal responses by symmetry (x = dim1 index, y = dim0 index)."""
+    img = np.zeros((64, 80), np.float32)
+    img[20:40, 30:60] = 1.0
+    x, y, s = oracle.harris(img, 500, 1e5, 1.0, 0, 0.04)
+    assert sorted(zip(x.tolist(), y.tolist())) == [(30.0, 20.0), (30.0, 39.0), (59.0, 20.0), (59.0, 39.0)]
+    assert np.ptp(s) <= 1e-6 * s.max()
+
+
+def test_harris_window_closed_forms():
+    g = oracle.harris_filter(7, 1.3, np.float64)
+    xs = np.arange(7) - 3
+    ref = np.exp(-xs ** 2 / (2 * 1.3 ** 2))
+    np.testing.assert_allclose(g, ref / ref.sum(), rtol=1e-14)
+    b = oracle.harris_filter(5, 0.2, np.float32)
+    assert np.all(b == np.float32(1) / np.float32(5))
+
+
+def test_harris_responses_match_fp64_formulation():
+    """The response map of the restatement (gradient, products, separable
+    smoothing, det - k tr^2) against scipy in FP64: every corner it reports
+    is a strict local maximum of the FP64 response, and the scores agree."""
+    import scipy.ndimage as ndi
+
+    rng = np.random.default_rng(3)
+    img = ndi.gaussian_filter(rng.random((90, 70)), 2.0)
+    h = oracle.harris_filter(5, 1.0, np.float64)
+    iy, ix = np.gradient(img)
+    sm = lambda a: ndi.convolve1d(ndi.convolve1d(a, h, axis=0, mode="constant"), h, axis=1, mode="constant")
+    ixx, ixy, iyy = sm(ix * ix), sm(ix * iy), sm(iy * iy)
+    r = ixx * iyy - ixy * ixy - 0.04 * (ixx + iyy) ** 2
+    x, y, s = oracle.harris(img, 0, 1e-30, 1.0, 0, 0.04)
+    assert len(x) > 5
+    for xi, yi, si in zip(x.astype(int), y.astype(int), s):
+        assert abs(r[yi, xi] - si) <= 1e-6 * np.abs(r).max()
+        nb = r[yi - 1:yi + 2, xi - 1:xi + 2].copy()
+        nb[1, 1] = -np.inf
+        assert r[yi, xi] > nb.max()
+
+
+def test_harris_top_k_is_stable_sort_of_detections():
+    rng = np.random.default_rng(4)
+    img = rng.random((60, 50)).astype(np.float32)
+    x0, y0, s0 = oracle.harris(img, 0, 1e-30, 1.0, 0, 0.04)   # min_r = 0, scan order
+    k = len(x0) // 3
+    x, y, s = oracle.harris(img, k, 0.0, 1.0, 0, 0.04)
+    order = sorted(range(len(s0)), key=lambda i: -float(s0[i]))[:k]
+    assert np.array_equal(x, x0[order]) and np.array_equal(y, y0[order]) and np.array_equal(s, s0[order])
+    xa, ya, sa = oracle.harris(img, 10 * len(x0), 0.0, 1.0, 0, 0.04)  # more asked than found: scan order
+    assert np.array_equal(xa, x0) and np.array_equal(sa, s0)
+
+
+@pytest.mark.parametrize("args,code", [
+    (((8, 8, 2), 10, 0.0, 1.0, 0, 0.04), 203),
+    (((32, 32), 0, 0.0, 1.0, 0, 0.04), 202),
+    (((32, 32), 10, 0.0, 1.0, 0, 0.001), 202),
+    (((32, 32), 10, 0.0, 7.0, 0, 0.04), 202),
+    (((32, 32), 10, 0.0, 1.0, 40, 0.04), 202),
+])
+def test_harris_argument_errors(args, code):
+    shape, *rest = args
+    with pytest.raises(ValueError) as e:
+        oracle.harris(np.zeros(shape, np.float32), *rest)
+    assert e.value.args[0] == code
