@@ -87,7 +87,8 @@ EXPORTED = [
     "afhip_k_iir_workspace", "afhip_k_iir", "afhip_k_conv2_nn_patch_noalloc_limit",
     "afhip_k_conv2_nn_patch_noalloc", "afhip_k_conv2_nn_dgrad_poly_workspace",
     "afhip_k_conv2_nn_dgrad_poly", "afhip_k_conv2_nn_gemm_workspace", "afhip_k_conv2_nn_gemm",
-    "afhip_k_conv2_nn_patch_tiles", "afhip_nn_step_plan_create", "afhip_nn_step_plan_run",
+    "afhip_k_conv2_nn_patch_tiles", "afhip_k_conv2_nn_gemm_split_workspace", "afhip_k_conv2_nn_gemm_split",
+    "afhip_nn_step_plan_create", "afhip_nn_step_plan_run",
     "afhip_nn_step_plan_release",
     # include/af/features.h, include/af/vision.h
     "af_create_features", "af_retain_features", "af_get_features_num", "af_get_features_xpos",

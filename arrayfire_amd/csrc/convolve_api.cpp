@@ -10,11 +10,14 @@
 #include <cstdlib>
 #include <memory>
 #include <vector>
+#include <string>
+#include <functional>
 
 #include "../../include/af/array.h"
 #include "../../include/af/ml.h"
 #include "../../include/af/signal.h"
 #include "../../include/afhip_kernels.h"
+#include "autotune.hpp"
 #include "runtime.hpp"
 
 namespace afhip {
@@ -209,26 +212,92 @@ static int device_cus() {
     return n;
 }
 
-// Runs the split gather GEMM (afhip_k_conv2_nn_gemm) when the patch kernels
-// would launch fewer workgroups than a quarter of the CUs and the GEMM splits
-// its reduction across them (small or strided layers). AFHIP_NN_SMALL=0 in
-// the environment keeps such layers on the patch kernels.
-static bool run_small(int pass, const ArrayImpl &in, const ArrayImpl &w, af_dtype out_type, int precision,
-                      void *out, const NNGeom &g) {
-    const char *env = std::getenv("AFHIP_NN_SMALL");
-    if (env && env[0] == '0') return false;
-    const long long tiles = afhip_k_conv2_nn_patch_tiles(pass, in.type, precision, g.x.d, g.w.d, g.y.d, g.st,
-                                                         g.pd, g.dl);
-    if (tiles > 0 && tiles * 4 >= device_cus()) return false;
-    const long long need = afhip_k_conv2_nn_gemm_workspace(pass, in.type, precision, g.x.d, g.w.d, g.y.d, g.st, g.pd, g.dl);
-    if (need <= 0) return false;
-    auto ws = scratch((size_t)need);
-    check_k(afhip_k_conv2_nn_gemm(pass, in.type, out_type, precision, out, in.data(), w.data(), g.x.d, g.w.d,
-                                  g.y.d, g.st, g.pd, g.dl, ws->ptr, need, stream()),
-            pass == 0 ? "convolve2_nn" : "convolve2_gradient_nn");
-    return true;
+// ---- NN variant choice. Layers whose patch kernels would leave most CUs
+// idle, and every layer the patch kernels do not serve, run the gather GEMM
+// with a split reduction; the autotuner (autotune.cpp) times the candidate
+// variants (the patch kernels where they serve; split counts 1, 2, 4, 8, 16)
+// on the first call of a geometry and caches the fastest under a key of
+// (pass, dtypes, precision, dims, stride, padding, dilation). Under stream
+// capture only cached choices are used. AFHIP_NN_AUTOTUNE=0 (or an explicit
+// AFHIP_NN_SPLIT) keeps the built-in rules.
+static bool nn_autotune_on() {
+    const char *a = std::getenv("AFHIP_NN_AUTOTUNE");
+    const char *sp = std::getenv("AFHIP_NN_SPLIT");
+    return !(a && a[0] == '0') && !(sp && sp[0] >= '1' && sp[0] <= '9');
 }
 
+static bool stream_capturing() {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(stream(), &st) == hipSuccess && st != hipStreamCaptureStatusNone;
+}
+
+// The key names the arithmetic (the MMA operand mode), not the storage
+// types, so that calls computing the same products (f32 arrays at BF16
+// precision and bf16 arrays; f32 and 16-bit outputs) run the same variant and
+// give the same sums.
+static std::string nn_key(const char *tag, int pass, const ArrayImpl &in, af_dtype, int precision,
+                          const NNGeom &g) {
+    const int mode = precision != AFHIP_PRECISION_DEFAULT ? precision
+                     : in.type == f32                     ? (int)AFHIP_PRECISION_F32
+                     : in.type == afhip_bf16              ? (int)AFHIP_PRECISION_BF16
+                                                          : (int)AFHIP_PRECISION_F16;
+    std::string k = std::string(tag) + ":p" + std::to_string(pass) + ":m" + std::to_string(mode);
+    for (int i = 0; i < 4; ++i) k += (i ? "," : ":x") + std::to_string(g.x[i]);
+    for (int i = 0; i < 4; ++i) k += (i ? "," : ":w") + std::to_string(g.w[i]);
+    k += ":s" + std::to_string(g.st[0]) + "," + std::to_string(g.st[1]) + ":pd" + std::to_string(g.pd[0]) + "," +
+         std::to_string(g.pd[1]) + ":d" + std::to_string(g.dl[0]) + "," + std::to_string(g.dl[1]);
+    return k;
+}
+
+// Index of the variant to run: the autotuner's (timing them now if the key
+// is new), or -1 for the built-in rule.
+static int nn_pick(const std::string &key, const std::vector<std::function<void()>> &cands) {
+    if (cands.size() < 2 || !nn_autotune_on()) return -1;
+    if (stream_capturing()) {
+        const int r = autotune_lookup(key);
+        return r >= 0 && r < (int)cands.size() ? r : -1;
+    }
+    return autotune_pick(key, (int)cands.size(), [&](int c) { cands[c](); });
+}
+
+static const int kSplitCands[] = {1, 2, 4, 8, 16};
+
+static void run_gemm_split(int pass, const ArrayImpl &in, const ArrayImpl &w, af_dtype out_type, int precision,
+                           void *out, const NNGeom &g, int nsplit) {
+    const long long need = afhip_k_conv2_nn_gemm_split_workspace(pass, in.type, precision, g.x.d, g.w.d, g.y.d,
+                                                                 g.st, g.pd, g.dl, nsplit);
+    std::shared_ptr<Buffer> ws;
+    if (need > 0) ws = scratch((size_t)need);
+    check_k(afhip_k_conv2_nn_gemm_split(pass, in.type, out_type, precision, out, in.data(), w.data(), g.x.d,
+                                        g.w.d, g.y.d, g.st, g.pd, g.dl, nsplit, ws ? ws->ptr : nullptr, need,
+                                        stream()),
+            pass == 0 ? "convolve2_nn" : "convolve2_gradient_nn");
+}
+
+static void run_patch_kernels(int pass, const ArrayImpl &in, const ArrayImpl &w, af_dtype out_type, int precision,
+                              void *out, const NNGeom &g, long long need) {
+    auto ws = scratch((size_t)need);
+    check_k(afhip_k_conv2_nn_patch(pass, in.type, out_type, precision, out, in.data(), w.data(), g.x.d, g.w.d,
+                                   g.y.d, g.st, g.pd, g.dl, ws->ptr, need, stream()),
+            pass == 0 ? "convolve2_nn" : "convolve2_gradient_nn");
+}
+
+// Candidate list: [patch kernels (when they serve)] + gather GEMM splits.
+static std::vector<std::function<void()>> nn_candidates(int pass, const ArrayImpl &in, const ArrayImpl &w,
+                                                        af_dtype out_type, int precision, void *out,
+                                                        const NNGeom &g, long long patch_need) {
+    std::vector<std::function<void()>> c;
+    if (patch_need > 0)
+        c.push_back([=, &in, &w, &g] { run_patch_kernels(pass, in, w, out_type, precision, out, g, patch_need); });
+    for (int sp : kSplitCands)
+        c.push_back([=, &in, &w, &g] { run_gemm_split(pass, in, w, out_type, precision, out, g, sp); });
+    return c;
+}
+
+// The patch kernels for layers that fill the chip with them; otherwise the
+// autotuned choice between them and the split gather GEMM (built-in rule:
+// the GEMM when the patch kernels launch fewer workgroups than a quarter of
+// the CUs). AFHIP_NN_SMALL=0 keeps every served layer on the patch kernels.
 static bool run_patch(int pass, const ArrayImpl &in, const ArrayImpl &w, af_dtype out_type,
                       int precision, void *out, const NNGeom &g) {
     const char *env = std::getenv("AFHIP_NN_GATHER");
@@ -236,24 +305,34 @@ static bool run_patch(int pass, const ArrayImpl &in, const ArrayImpl &w, af_dtyp
     const long long need = afhip_k_conv2_nn_patch_workspace(pass, in.type, precision, g.x.d, g.w.d,
                                                             g.y.d, g.st, g.pd, g.dl);
     if (need <= 0) return false;
-    if (run_small(pass, in, w, out_type, precision, out, g)) return true;
-    auto ws = scratch((size_t)need);
-    check_k(afhip_k_conv2_nn_patch(pass, in.type, out_type, precision, out, in.data(), w.data(), g.x.d,
-                                   g.w.d, g.y.d, g.st, g.pd, g.dl, ws->ptr, need, stream()),
-            pass == 0 ? "convolve2_nn" : "convolve2_gradient_nn");
+    const char *sm = std::getenv("AFHIP_NN_SMALL");
+    const long long tiles = afhip_k_conv2_nn_patch_tiles(pass, in.type, precision, g.x.d, g.w.d, g.y.d, g.st,
+                                                         g.pd, g.dl);
+    const int cus = device_cus();
+    if ((sm && sm[0] == '0') || tiles >= cus) {
+        run_patch_kernels(pass, in, w, out_type, precision, out, g, need);
+        return true;
+    }
+    const auto cands = nn_candidates(pass, in, w, out_type, precision, out, g, need);
+    int c = nn_pick(nn_key("nn", pass, in, out_type, precision, g), cands);
+    if (c < 0) c = (tiles > 0 && tiles * 4 >= cus) ? 0 : -1;  // the rule
+    if (c >= 0)
+        cands[c]();
+    else
+        run_gemm_split(pass, in, w, out_type, precision, out, g, 0);
     return true;
 }
 
-// The gather GEMM for every geometry, its reduction split when that fills
-// the chip (workspace from the pool).
+// The gather GEMM for every geometry, its split count autotuned (built-in
+// rule: split when that fills the chip).
 static void run_gemm(int pass, const ArrayImpl &in, const ArrayImpl &w, af_dtype out_type, int precision,
                      void *out, const NNGeom &g) {
-    const long long need = afhip_k_conv2_nn_gemm_workspace(pass, in.type, precision, g.x.d, g.w.d, g.y.d, g.st, g.pd, g.dl);
-    std::shared_ptr<Buffer> ws;
-    if (need > 0) ws = scratch((size_t)need);
-    check_k(afhip_k_conv2_nn_gemm(pass, in.type, out_type, precision, out, in.data(), w.data(), g.x.d, g.w.d,
-                                  g.y.d, g.st, g.pd, g.dl, ws ? ws->ptr : nullptr, need, stream()),
-            pass == 0 ? "convolve2_nn" : "convolve2_gradient_nn");
+    const auto cands = nn_candidates(pass, in, w, out_type, precision, out, g, 0);
+    const int c = nn_pick(nn_key("nn_gemm", pass, in, out_type, precision, g), cands);
+    if (c >= 0)
+        cands[c]();
+    else
+        run_gemm_split(pass, in, w, out_type, precision, out, g, 0);
 }
 
 static af_dtype nn_out_type(af_dtype in, af_dtype requested) {

@@ -78,6 +78,7 @@ struct NNParams {
     int cl;
     int Rcp;
     int wp_bytes, act_bytes;  // sizes of the channel-last copies (each below 2^31 - 64)
+    int nsplit_req;           // > 0: the caller's split count (forward / data gradient)
 };
 
 // Phase extents: pixels of residue r along an axis of length L and stride s,
@@ -825,6 +826,7 @@ static int setup(NNParams &p, int pass, const long long *xd, const long long *wd
     p.cl = 0;
     p.Rcp = 0;
     p.wp_bytes = p.act_bytes = 0;
+    p.nsplit_req = 0;
     for (int i = 0; i <= kMaxPhases; ++i) p.ph_tile0[i] = 0;
     if (pass == P_FWD) {
         p.M = p.K;
@@ -930,6 +932,8 @@ static int gemm_splits(const NNParams &p) {
     const char *env = std::getenv("AFHIP_NN_SPLIT");
     if (env && env[0] >= '1' && env[0] <= '9') {
         want = std::atoll(env);
+    } else if (p.nsplit_req > 0) {
+        want = p.nsplit_req;
     } else {
         const long long cus = cu_count_current();
         if (tiles * 2 >= cus) return 1;
@@ -1057,14 +1061,16 @@ static int dispatch(int pass, int in_type, int out_type, int precision, void *ou
 
 using namespace afhip;
 
-extern "C" long long afhip_k_conv2_nn_gemm_workspace(int pass, int in_type, int precision, const long long *xdims,
-                                                     const long long *wdims, const long long *ydims,
-                                                     const long long *stride, const long long *pad,
-                                                     const long long *dil) {
+extern "C" long long afhip_k_conv2_nn_gemm_split_workspace(int pass, int in_type, int precision,
+                                                           const long long *xdims, const long long *wdims,
+                                                           const long long *ydims, const long long *stride,
+                                                           const long long *pad, const long long *dil,
+                                                           int nsplit) {
     if (pass != P_FWD && pass != P_DGRAD) return 0;
     NNParams p;
     if (setup(p, pass, xdims, wdims, ydims, stride, pad, dil)) return 0;
     if (p.M == 0 || p.Ncol == 0) return 0;
+    p.nsplit_req = nsplit > 0 ? nsplit : 0;
     // channel-last path: operands stored at their own precision
     const bool cl = (in_type == DT_F32 && (precision == 0 || precision == 4)) ||
                     (in_type == DT_BF16 && (precision == 0 || precision == 1)) ||
@@ -1073,17 +1079,35 @@ extern "C" long long afhip_k_conv2_nn_gemm_workspace(int pass, int in_type, int 
     return gemm_layout(p, pass, esz, gemm_splits(p)).total;
 }
 
+extern "C" int afhip_k_conv2_nn_gemm_split(int pass, int in_type, int out_type, int precision, void *out,
+                                           const void *in, const void *w, const long long *xdims,
+                                           const long long *wdims, const long long *ydims,
+                                           const long long *stride, const long long *pad, const long long *dil,
+                                           int nsplit, void *workspace, long long workspace_bytes, void *stream) {
+    if (pass != P_FWD && pass != P_DGRAD) return K_ERR_ARG;
+    NNParams p;
+    int rc = setup(p, pass, xdims, wdims, ydims, stride, pad, dil);
+    if (rc) return rc;
+    p.nsplit_req = nsplit > 0 ? nsplit : 0;
+    return dispatch(pass, in_type, out_type, precision, out, in, w, p, workspace, workspace_bytes,
+                    (hipStream_t)stream);
+}
+
+extern "C" long long afhip_k_conv2_nn_gemm_workspace(int pass, int in_type, int precision, const long long *xdims,
+                                                     const long long *wdims, const long long *ydims,
+                                                     const long long *stride, const long long *pad,
+                                                     const long long *dil) {
+    return afhip_k_conv2_nn_gemm_split_workspace(pass, in_type, precision, xdims, wdims, ydims, stride, pad, dil,
+                                                 0);
+}
+
 extern "C" int afhip_k_conv2_nn_gemm(int pass, int in_type, int out_type, int precision, void *out,
                                      const void *in, const void *w, const long long *xdims,
                                      const long long *wdims, const long long *ydims,
                                      const long long *stride, const long long *pad, const long long *dil,
                                      void *workspace, long long workspace_bytes, void *stream) {
-    if (pass != P_FWD && pass != P_DGRAD) return K_ERR_ARG;
-    NNParams p;
-    int rc = setup(p, pass, xdims, wdims, ydims, stride, pad, dil);
-    if (rc) return rc;
-    return dispatch(pass, in_type, out_type, precision, out, in, w, p, workspace, workspace_bytes,
-                    (hipStream_t)stream);
+    return afhip_k_conv2_nn_gemm_split(pass, in_type, out_type, precision, out, in, w, xdims, wdims, ydims, stride,
+                                       pad, dil, 0, workspace, workspace_bytes, stream);
 }
 
 extern "C" int afhip_k_conv2_nn_fwd(int in_type, int out_type, int precision, void *y,

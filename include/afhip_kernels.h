@@ -145,6 +145,25 @@ AFHIP_KAPI int afhip_k_conv2_nn_gemm(int pass, int in_type, int out_type,
                                      const long long *dil, void *workspace,
                                      long long workspace_bytes, void *stream);
 
+/* afhip_k_conv2_nn_gemm with the reduction split nsplit ways (nsplit <= 0:
+ * the built-in rule), for the autotuner; the query sizes that workspace. */
+AFHIP_KAPI long long afhip_k_conv2_nn_gemm_split_workspace(
+    int pass, int in_type, int precision, const long long *xdims,
+    const long long *wdims, const long long *ydims, const long long *stride,
+    const long long *pad, const long long *dil, int nsplit);
+AFHIP_KAPI int afhip_k_conv2_nn_gemm_split(int pass, int in_type, int out_type,
+                                           int precision, void *out,
+                                           const void *in, const void *w,
+                                           const long long *xdims,
+                                           const long long *wdims,
+                                           const long long *ydims,
+                                           const long long *stride,
+                                           const long long *pad,
+                                           const long long *dil, int nsplit,
+                                           void *workspace,
+                                           long long workspace_bytes,
+                                           void *stream);
+
 /* Workspace bytes afhip_k_conv2_nn_wgrad uses for its split reduction over
  * N*OH*OW (0 when it runs unsplit). */
 AFHIP_KAPI long long afhip_k_conv2_nn_wgrad_workspace(
