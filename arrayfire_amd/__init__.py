@@ -96,7 +96,9 @@ EXPORTED = [
     "af_get_features_size", "af_release_features", "af_harris",
     "afhip_k_harris_deriv", "afhip_k_harris_nms", "afhip_k_select_flagged_workspace",
     "afhip_k_select_flagged", "afhip_k_sort_desc_workspace", "afhip_k_harris_sort",
-    "afhip_k_harris_out",
+    "afhip_k_harris_out", "af_canny", "afhip_k_canny_grad", "afhip_k_canny_nms",
+    "afhip_k_canny_minmax", "afhip_k_canny_otsu", "afhip_k_canny_candidates", "afhip_k_canny_seed",
+    "afhip_k_canny_sweep",
 ]
 
 
@@ -173,6 +175,8 @@ def lib() -> ctypes.CDLL:
         for n in ("xpos", "ypos", "score", "orientation", "size"):
             getattr(L, "af_get_features_" + n).argtypes = [ctypes.POINTER(vp), vp]
         L.af_release_features.argtypes = [vp]
+        L.af_canny.argtypes = [ctypes.POINTER(vp), vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, u,
+                               ctypes.c_bool]
         L.afhip_nn_step_plan_release.argtypes = [vp]
         ll = ctypes.c_longlong
         L.afhip_k_convolve.argtypes = [ctypes.c_int, vp, i64p, vp, i64p, vp, i64p, ctypes.c_int,
@@ -569,6 +573,14 @@ def harris(image, max_corners: int = 500, min_response: float = 1e5, sigma: floa
     check(lib().af_harris(ctypes.byref(h), x.handle, int(max_corners), float(min_response),
                           float(sigma), int(block_size), float(k_thr)))
     return Features(h.value)
+
+
+def canny(image, threshold_type: int = 0, low: float = 0.1, high: float = 0.3, sobel_window: int = 3,
+          is_fast: bool = False) -> Array:
+    """af::canny (include/af/image.h:735): a b8 edge map."""
+    x = _arr(image)
+    return _out(lib().af_canny, x.handle, int(threshold_type), float(low), float(high), int(sobel_window),
+                bool(is_fast))
 
 
 def convolve2_ragged(signals, filters, modes) -> list:

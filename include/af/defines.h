@@ -81,6 +81,12 @@ typedef enum {
     AF_CONV_FREQ = 2
 } af_conv_domain;
 
+/* include/af/defines.h:486-489 */
+typedef enum {
+    AF_CANNY_THRESHOLD_MANUAL = 0,
+    AF_CANNY_THRESHOLD_AUTO_OTSU = 1
+} af_canny_threshold;
+
 typedef enum {
     AF_CONV_GRADIENT_DEFAULT = 0,
     AF_CONV_GRADIENT_FILTER = 1,

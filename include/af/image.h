@@ -35,3 +35,14 @@ AFAPI af_err af_gaussian_kernel(af_array *out, const int rows, const int cols,
  * input); 2-D or 3-D input (the third dim is a batch). */
 AFAPI af_err af_dog(af_array *out, const af_array in, const int radius1,
                     const int radius2);
+/* include/af/image.h:1683; src/api/c/canny.cpp:224-285. Binary (b8) edge
+ * map of a 2-D image (or a batch of them) of f32, f64, s32, u32, s16, u16,
+ * s8 or u8: 5-tap smoothing, 3x3 Sobel (sobel_window must be 3), L1
+ * (is_fast) or L2 magnitude, non-maximum suppression, then hysteresis with
+ * thresholds low/high of the normalised magnitude (MANUAL) or low * Otsu's
+ * threshold / Otsu's threshold (AUTO_OTSU, high ignored). */
+AFAPI af_err af_canny(af_array *out, const af_array in,
+                      const af_canny_threshold threshold_type,
+                      const float low_threshold_ratio,
+                      const float high_threshold_ratio,
+                      const unsigned sobel_window, const bool is_fast);

@@ -429,3 +429,40 @@ AFHIP_KAPI int afhip_k_harris_out(float *x, float *y, float *score,
                                   const unsigned *sel, const unsigned *order,
                                   const float *sorted_keys, const float *resp,
                                   long long W, long long n, void *stream);
+
+/* Canny edge detector kernels (af_canny, src/api/c/canny.cpp:198-285;
+ * src/backend/cpu/kernel/sobel.hpp, canny.hpp, histogram.hpp). Images are
+ * ns slices of d0 x d1 f32 values. grad: Sobel dx (dim0) / dy (dim1), reflect101
+ * borders, and the L1 (is_fast) or L2 magnitude. nms: the suppressed map. */
+AFHIP_KAPI int afhip_k_canny_grad(float *gx, float *gy, float *mag,
+                                  const float *smooth, long long d0,
+                                  long long d1, long long ns, int is_fast,
+                                  void *stream);
+AFHIP_KAPI int afhip_k_canny_nms(float *out, const float *mag, const float *gx,
+                                 const float *gy, long long d0, long long d1,
+                                 long long ns, void *stream);
+/* mm[0] = min, mm[1] = max of n values. */
+AFHIP_KAPI int afhip_k_canny_minmax(float *mm, const float *v, long long n,
+                                    void *stream);
+/* Per slice: the histogram of nbins over [0, maxval] into hist (zeroed by
+ * the caller) and Otsu's threshold (a bin index) into thr. */
+AFHIP_KAPI int afhip_k_canny_otsu(float *thr, unsigned *hist, const float *v,
+                                  long long per, long long ns, int nbins,
+                                  float maxval, void *stream);
+/* Strong / weak maps: mode 0 normalises by mm and compares with t1, t2;
+ * mode 1 compares with thr[slice] * t1 and thr[slice]. */
+AFHIP_KAPI int afhip_k_canny_candidates(unsigned char *strong,
+                                        unsigned char *weak, const float *v,
+                                        const float *mm, const float *thr,
+                                        long long per, long long ns, float t1,
+                                        float t2, int mode, void *stream);
+/* Hysteresis: seeds (strong pixels at the reference's scan positions), then
+ * sweeps marking weak pixels next to marked ones; *changed is set by a
+ * sweep that marks anything. */
+AFHIP_KAPI int afhip_k_canny_seed(unsigned char *out,
+                                  const unsigned char *strong, long long d0,
+                                  long long d1, long long ns, void *stream);
+AFHIP_KAPI int afhip_k_canny_sweep(unsigned char *out,
+                                   const unsigned char *weak, long long d0,
+                                   long long d1, long long ns,
+                                   unsigned *changed, void *stream);

@@ -805,3 +805,189 @@ def harris(img: np.ndarray, max_corners: int, min_response: float, sigma: float,
         order = order[:corners_out]
         return xs[order], ys[order], rs[order]
     return xs[:corners_out], ys[:corners_out], rs[:corners_out]
+
+
+# ------------------------------------------------------------------ Canny
+CANNY_SMOOTH = np.array([-0.11021, -0.23691, -0.30576, -0.23691, -0.11021], dtype=np.float32)
+
+
+def sobel3(x: np.ndarray):
+    """cpu::kernel::derivative (src/backend/cpu/kernel/sobel.hpp:20-70) for
+    one float32 slice, reflect101 borders: (dx along dim0, dy along dim1)."""
+    d0, d1 = x.shape
+    r = lambda idx, end: abs(end - abs(end - idx))
+    i = np.arange(d0)
+    j = np.arange(d1)
+    im = np.array([r(v - 1, d0 - 1) for v in i]); ip = np.array([r(v + 1, d0 - 1) for v in i])
+    jm = np.array([r(v - 1, d1 - 1) for v in j]); jp = np.array([r(v + 1, d1 - 1) for v in j])
+    NW = x[np.ix_(im, jm)]; SW = x[np.ix_(ip, jm)]; NE = x[np.ix_(im, jp)]; SE = x[np.ix_(ip, jp)]
+    N = x[np.ix_(im, j)]; S = x[np.ix_(ip, j)]; W = x[np.ix_(i, jm)]; E = x[np.ix_(i, jp)]
+    two = np.float32(2)
+    dx = ((SW + SE) - (NW + NE)) + two * (S - N)
+    dy = ((NE + SE) - (NW + SW)) + two * (E - W)
+    return dx, dy
+
+
+def canny_nms(mag: np.ndarray, gx: np.ndarray, gy: np.ndarray) -> np.ndarray:
+    """cpu::kernel::nonMaxSuppression (src/backend/cpu/kernel/canny.hpp:18-110)
+    for one slice: interior pixels whose magnitude exceeds both values
+    interpolated along the gradient keep it, others (and the border) 0."""
+    d0, d1 = mag.shape
+    out = np.zeros_like(mag)
+    one = np.float32(1)
+    for j in range(1, d1 - 1):
+        for i in range(1, d0 - 1):
+            curr = mag[i, j]
+            if curr == 0:
+                continue
+            se, nw = mag[i + 1, j + 1], mag[i - 1, j - 1]
+            ea, we = mag[i + 1, j], mag[i - 1, j]
+            ne, sw = mag[i + 1, j - 1], mag[i - 1, j + 1]
+            no, so = mag[i, j - 1], mag[i, j + 1]
+            dx, dy = gx[i, j], gy[i, j]
+            if dx >= 0:
+                if dy >= 0:
+                    g = (dx - dy) >= 0
+                    a1, a2, b1, b2 = (ea if g else so), (we if g else no), se, nw
+                    alpha = dy / dx if g else dx / dy
+                else:
+                    g = (dx + dy) >= 0
+                    a1, a2, b1, b2 = (ea if g else no), (we if g else so), ne, sw
+                    alpha = -dy / dx if g else dx / -dy
+            else:
+                if dy >= 0:
+                    g = (dx + dy) >= 0
+                    a1, a2, b1, b2 = (so if g else we), (no if g else ea), sw, ne
+                    alpha = -dx / dy if g else dy / -dx
+                else:
+                    g = (-dx + dy) >= 0
+                    a1, a2, b1, b2 = (we if g else no), (ea if g else so), nw, se
+                    alpha = dy / dx if g else dx / dy
+            with np.errstate(all="ignore"):
+                m1 = (one - alpha) * a1 + alpha * b1
+                m2 = (one - alpha) * a2 + alpha * b2
+            if curr > m1 and curr > m2:
+                out[i, j] = curr
+    return out
+
+
+def _otsu_threshold(sup: np.ndarray, nbins: int, maxval: np.float32) -> np.float32:
+    """otsuThreshold (src/api/c/canny.cpp:82-147) for one slice: histogram
+    (src/backend/cpu/kernel/histogram.hpp:19-47) over [0, maxval], scans and
+    sums in float in bin order, the index of the first largest between-class
+    variance (ireduce max: NaN never wins) as the threshold."""
+    f32 = np.float32
+    step = f32((float(maxval) - 0.0) / float(f32(nbins)))
+    hist = np.zeros(nbins, np.int64)
+    for v in sup.ravel(order="F"):
+        b = int((f32(v) - f32(0.0)) / step) if np.isfinite((f32(v) - f32(0.0)) / step) else 0
+        b = min(max(b, 0), nbins - 1)
+        hist[b] += 1
+    total = f32(sup.shape[0] * sup.shape[1])
+    freqs = hist.astype(f32) / total
+    cum = np.zeros(nbins, f32)
+    cprod = np.zeros(nbins, f32)
+    s = f32(0); sp = f32(0); ws = f32(0)
+    for b in range(nbins):
+        s = f32(s + freqs[b]); cum[b] = s
+        p = f32(f32(b) * freqs[b])
+        sp = f32(sp + p); cprod[b] = sp
+        ws = f32(p + ws)
+    sig = np.zeros(max(nbins - 1, 0), f32)
+    with np.errstate(all="ignore"):
+        for b in range(nbins - 1):
+            qL = cum[b]
+            qH = f32(f32(1) - qL)
+            muL = f32(cprod[b] / qL)
+            muH = f32(f32(ws - cprod[b]) / qH)
+            diff = f32(muL - muH)
+            sig[b] = f32(f32(diff * diff) * f32(qL * qH))
+    # MinMaxOp<af_max_t> (src/backend/cpu/kernel/ireduce.hpp:47-63): starts
+    # from element 0 (a NaN there becomes the lowest float, whose magnitude
+    # only an infinity beats), takes larger magnitudes, keeps the first of
+    # equal ones; NaN never compares larger
+    if sig.size == 0:
+        return f32(0)
+    best_v = sig[0] if not np.isnan(sig[0]) else f32(np.finfo(np.float32).min)
+    best_i = 0
+    for b in range(1, sig.size):
+        if abs(sig[b]) > abs(best_v):
+            best_v, best_i = sig[b], b
+    return f32(best_i)
+
+
+def _hysteresis(strong: np.ndarray, weak: np.ndarray) -> np.ndarray:
+    """cpu::kernel::edgeTrackingHysteresis (src/backend/cpu/kernel/canny.hpp:
+    112-186) for one slice: seeds are the strong pixels at linear indices
+    d0+1 .. d0+1+(d0-1)*(d1-1)-1 (the reference's scan), each traced through
+    8-neighbours (linear offsets) that are weak."""
+    d0, d1 = strong.shape
+    s = strong.ravel(order="F"); w = weak.ravel(order="F")
+    out = np.zeros(s.size, np.uint8)
+    n = s.size
+    t = d0 + 1
+    for _ in range((d0 - 1) * (d1 - 1)):
+        if s[t] > 0 and out[t] != 1:
+            out[t] = 1
+            q = [t]
+            while q:
+                c = q.pop()
+                for it in (c - d0 - 1, c - d0, c - d0 + 1, c - 1, c + 1, c + d0 - 1, c + d0, c + d0 + 1):
+                    if 0 <= it < n and w[it] > 0 and out[it] != 1:
+                        out[it] = 1
+                        q.append(it)
+        t += 1
+    return out.reshape((d0, d1), order="F")
+
+
+def canny(img: np.ndarray, threshold_type: int, t1: float, t2: float, sw: int = 3,
+          is_fast: bool = False) -> np.ndarray:
+    """af_canny (src/api/c/canny.cpp:212-285): 5-tap separable smoothing in
+    float, 3x3 Sobel, L1 or L2 magnitude, non-maximum suppression, weak and
+    strong candidates (MANUAL: normalised by the suppressed map's min and
+    max; AUTO_OTSU: Otsu's threshold per slice), edge tracking by hysteresis.
+    Returns uint8 0/1 of the input's dims. Raises ValueError(af_err)."""
+    x = np.asarray(img)
+    if x.ndim < 2 or (x.ndim >= 2 and x.shape[1] == 1):
+        raise ValueError(203)
+    if not (x.shape[0] >= 5 and x.shape[1] >= 5):
+        raise ValueError(203)
+    if sw != 3:
+        raise ValueError(202)
+    if x.dtype not in (np.float32, np.float64, np.int32, np.uint32, np.int16, np.uint16, np.int8, np.uint8):
+        raise ValueError(204)
+    x4 = as4(x.astype(np.float32))
+    smt = convolve2_sep(CANNY_SMOOTH, CANNY_SMOOTH, x4, False)
+    d0, d1, d2, d3 = smt.shape
+    sup = np.zeros_like(smt)
+    for b3 in range(d3):
+        for b2 in range(d2):
+            gx, gy = sobel3(smt[:, :, b2, b3])
+            if is_fast:
+                mag = np.abs(gx) + np.abs(gy)
+            else:
+                mag = np.sqrt(gx * gx + gy * gy)
+            with np.errstate(all="ignore"):
+                sup[:, :, b2, b3] = canny_nms(mag, gx, gy)
+    maxv = np.float32(sup.max())
+    out = np.zeros(sup.shape, np.uint8)
+    if threshold_type == 1:
+        nbins = int(maxv)
+        for b3 in range(d3):
+            for b2 in range(d2):
+                s = sup[:, :, b2, b3]
+                T2 = _otsu_threshold(s, nbins, maxv)
+                T1 = np.float32(T2 * np.float32(t1))
+                weak = (s >= T1) & (s < T2)
+                strong = s >= T2
+                out[:, :, b2, b3] = _hysteresis(strong, weak)
+    else:
+        minv = np.float32(sup.min())
+        with np.errstate(all="ignore"):
+            norm = (sup - minv) / np.float32(maxv - minv)
+        weak = (norm >= np.float32(t1)) & (norm < np.float32(t2))
+        strong = norm >= np.float32(t2)
+        for b3 in range(d3):
+            for b2 in range(d2):
+                out[:, :, b2, b3] = _hysteresis(strong[:, :, b2, b3], weak[:, :, b2, b3])
+    return out

@@ -581,3 +581,81 @@ def test_harris_argument_errors(args, code):
     with pytest.raises(ValueError) as e:
         oracle.harris(np.zeros(shape, np.float32), *rest)
     assert e.value.args[0] == code
+
+
+# ------------------------------------------------------------------ Canny
+def test_canny_sobel_is_scipy_sobel_mirror():
+    import scipy.ndimage as ndi
+
+    x = np.random.default_rng(5).random((23, 31)).astype(np.float32)
+    dx, dy = oracle.sobel3(x)
+    np.testing.assert_allclose(dx, ndi.sobel(x.astype(np.float64), axis=0, mode="mirror"), atol=1e-5)
+    np.testing.assert_allclose(dy, ndi.sobel(x.astype(np.float64), axis=1, mode="mirror"), atol=1e-5)
+
+
+def test_canny_rectangle_edges_on_boundary():
+    img = np.zeros((48, 64), np.float32)
+    img[12:36, 16:48] = 255.0
+    e = oracle.canny(img, 0, 0.1, 0.3)[..., 0, 0]
+    ys, xs = np.nonzero(e)
+    assert len(ys) > 60
+    # every edge pixel lies within one pixel of the rectangle's boundary
+    dist = np.minimum.reduce([np.abs(ys - 11.5), np.abs(ys - 35.5), np.abs(xs - 15.5), np.abs(xs - 47.5)])
+    inside = (ys >= 10) & (ys <= 37) & (xs >= 14) & (xs <= 49)
+    assert np.all(inside) and np.all(dist <= 1.5)
+
+
+def test_canny_hysteresis_is_seeded_components():
+    """Edge tracking marks exactly the 8-connected components of
+    (weak | seeds) that hold a seed (pixels off the image border)."""
+    import scipy.ndimage as ndi
+
+    rng = np.random.default_rng(6)
+    strong = rng.random((30, 25)) > 0.97
+    weak = (rng.random((30, 25)) > 0.55) & ~strong
+    strong[0, :] = strong[-1, :] = strong[:, 0] = strong[:, -1] = False
+    weak[0, :] = weak[-1, :] = weak[:, 0] = weak[:, -1] = False
+    out = oracle._hysteresis(strong, weak)
+    lab, _ = ndi.label(weak | strong, structure=np.ones((3, 3)))
+    seeded = np.unique(lab[strong])
+    ref = np.isin(lab, seeded[seeded > 0])
+    np.testing.assert_array_equal(out.astype(bool), ref)
+
+
+def test_canny_otsu_matches_between_class_variance():
+    """The threshold is the argmax of the between-class variance
+    q0 q1 (mu0 - mu1)^2 over the histogram (FP64 evaluation)."""
+    rng = np.random.default_rng(7)
+    sup = np.concatenate([rng.normal(20, 3, 400), rng.normal(70, 5, 300), np.zeros(300)]).clip(0).astype(np.float32)
+    sup = sup.reshape(25, 40)
+    maxv = np.float32(sup.max())
+    nb = int(maxv)
+    t = oracle._otsu_threshold(sup, nb, maxv)
+    step = float(maxv) / nb
+    hist = np.bincount(np.clip((sup.ravel() / step).astype(int), 0, nb - 1), minlength=nb).astype(np.float64)
+    p = hist / hist.sum()
+    w = np.arange(nb)
+    best, bi = -1, 0
+    for b in range(nb - 1):
+        q0 = p[:b + 1].sum()
+        q1 = 1 - q0
+        if q0 <= 0 or q1 <= 0:
+            continue
+        m0 = (w[:b + 1] * p[:b + 1]).sum() / q0
+        m1 = (w[b + 1:] * p[b + 1:]).sum() / q1
+        v = q0 * q1 * (m0 - m1) ** 2
+        if v > best + 1e-9 * abs(best):
+            best, bi = v, b
+    assert abs(t - bi) <= 1
+
+
+@pytest.mark.parametrize("args,code", [
+    (((4, 8), 0, 0.1, 0.3, 3), 203),
+    (((16, 16), 0, 0.1, 0.3, 5), 202),
+    (((16,), 0, 0.1, 0.3, 3), 203),
+])
+def test_canny_argument_errors(args, code):
+    shape, *rest = args
+    with pytest.raises(ValueError) as e:
+        oracle.canny(np.zeros(shape, np.float32), *rest)
+    assert e.value.args[0] == code
