@@ -98,7 +98,8 @@ EXPORTED = [
     "afhip_k_select_flagged", "afhip_k_sort_desc_workspace", "afhip_k_harris_sort",
     "afhip_k_harris_out", "af_canny", "afhip_k_canny_grad", "afhip_k_canny_nms",
     "afhip_k_canny_minmax", "afhip_k_canny_otsu", "afhip_k_canny_candidates", "afhip_k_canny_seed",
-    "afhip_k_canny_sweep",
+    "afhip_k_canny_sweep", "af_orb", "afhip_k_orb_resize", "afhip_k_orb_fast", "afhip_k_orb_fast_nonmax",
+    "afhip_k_orb_harris", "afhip_k_orb_angle", "afhip_k_orb_describe",
 ]
 
 
@@ -177,6 +178,8 @@ def lib() -> ctypes.CDLL:
         L.af_release_features.argtypes = [vp]
         L.af_canny.argtypes = [ctypes.POINTER(vp), vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, u,
                                ctypes.c_bool]
+        L.af_orb.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(vp), vp, ctypes.c_float, u, ctypes.c_float, u,
+                             ctypes.c_bool]
         L.afhip_nn_step_plan_release.argtypes = [vp]
         ll = ctypes.c_longlong
         L.afhip_k_convolve.argtypes = [ctypes.c_int, vp, i64p, vp, i64p, vp, i64p, ctypes.c_int,
@@ -581,6 +584,18 @@ def canny(image, threshold_type: int = 0, low: float = 0.1, high: float = 0.3, s
     x = _arr(image)
     return _out(lib().af_canny, x.handle, int(threshold_type), float(low), float(high), int(sobel_window),
                 bool(is_fast))
+
+
+def orb(image, fast_thr: float = 20.0, max_feat: int = 400, scl_fctr: float = 1.5, levels: int = 4,
+        blur_img: bool = False):
+    """af::orb (include/af/vision.h:429): (Features, descriptors), the
+    descriptors a u32 Array of dims [8, n]."""
+    x = _arr(image)
+    h = ctypes.c_void_p()
+    d = ctypes.c_void_p()
+    check(lib().af_orb(ctypes.byref(h), ctypes.byref(d), x.handle, float(fast_thr), int(max_feat), float(scl_fctr),
+                       int(levels), bool(blur_img)))
+    return Features(h.value), Array(handle=d.value)
 
 
 def convolve2_ragged(signals, filters, modes) -> list:

@@ -12,14 +12,10 @@
 #include "../../include/af/features.h"
 #include "../../include/af/vision.h"
 #include "../../include/afhip_kernels.h"
+#include "features.hpp"
 #include "runtime.hpp"
 
 namespace afhip {
-
-struct Features {
-    dim_t n = 0;
-    af_array x = nullptr, y = nullptr, score = nullptr, orientation = nullptr, size = nullptr;
-};
 
 static Features &feat(const af_features f) {
     if (!f) AFHIP_THROW(AF_ERR_ARG, "Invalid features handle");

@@ -466,3 +466,47 @@ AFHIP_KAPI int afhip_k_canny_sweep(unsigned char *out,
                                    const unsigned char *weak, long long d0,
                                    long long d1, long long ns,
                                    unsigned *changed, void *stream);
+
+/* ORB kernels (af_orb, src/api/c/orb.cpp:58-89; src/backend/cpu/orb.cpp:44-283;
+ * src/backend/cpu/kernel/orb.hpp, fast.hpp, resize.hpp). f32 or f64 images
+ * of d0 x d1. */
+/* Bilinear resize of one id0 x id1 slice to od0 x od1 (resize.hpp:86-130). */
+AFHIP_KAPI int afhip_k_orb_resize(int dtype, void *out, const void *in,
+                                  long long id0, long long id1, long long od0,
+                                  long long od1, void *stream);
+/* FAST test of every pixel at least `edge` from the borders: flags and
+ * scores at the pixel's scan rank (y - edge) * (d1 - 2 edge) + (x - edge). */
+AFHIP_KAPI int afhip_k_orb_fast(int dtype, unsigned char *flags, float *score,
+                                const void *img, long long d0, long long d1,
+                                int edge, float thr, int arc, void *stream);
+/* Suppression of the n corners sel[] (scan ranks) on the score map V
+ * (d0 x d1 floats, cleared here): flags[j] keeps corner j. */
+AFHIP_KAPI int afhip_k_orb_fast_nonmax(unsigned char *flags, float *V,
+                                       const unsigned *sel, const float *score,
+                                       long long n, long long d0, long long d1,
+                                       int edge, void *stream);
+/* Harris response of the n corners sel1[sel2[j]] over a block x block
+ * window; flags[j] = 0 for corners within patch_r of a border. */
+AFHIP_KAPI int afhip_k_orb_harris(int dtype, unsigned char *flags, float *resp,
+                                  const void *img, const unsigned *sel1,
+                                  const unsigned *sel2, long long n,
+                                  long long d0, long long d1, int edge,
+                                  int patch_r, int block, float k_thr,
+                                  void *stream);
+/* The n kept features sel1[sel2[sel3[order[f]]]] with scores keys[f]:
+ * positions and intensity-centroid angle over a patch x patch window. */
+AFHIP_KAPI int afhip_k_orb_angle(int dtype, float *x, float *y, float *score,
+                                 float *ori, const void *img,
+                                 const unsigned *sel1, const unsigned *sel2,
+                                 const unsigned *sel3, const unsigned *order,
+                                 const float *keys, long long n, long long d0,
+                                 long long d1, int edge, int patch,
+                                 void *stream);
+/* Rotated-BRIEF descriptors (8 words per feature) sampled from img, and the
+ * positions and sizes scaled by scl. */
+AFHIP_KAPI int afhip_k_orb_describe(int dtype, unsigned *desc, float *ox,
+                                    float *oy, float *osize, const float *x,
+                                    const float *y, const float *ori,
+                                    const void *img, long long n, long long d0,
+                                    long long d1, int patch, int r, float scl,
+                                    void *stream);

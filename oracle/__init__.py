@@ -21,6 +21,11 @@ Two families of functions live here:
   - ``conv2_data_grad``      ... src/backend/cpu/convolve.cpp:179-211
   - ``conv2_filter_grad``    ... src/backend/cpu/convolve.cpp:213-243
   - ``gaussian_kernel``      ... src/api/c/gaussian_kernel.cpp:35-88
+  - ``harris``               ... src/backend/cpu/harris.cpp:26-148
+  - ``canny``                ... src/api/c/canny.cpp:198-285
+  - ``orb`` (``fast``, ``resize_bilinear``)
+                             ... src/backend/cpu/orb.cpp:44-283, kernel/orb.hpp,
+                                 kernel/fast.hpp, kernel/resize.hpp:86-130
 
   The SGEMM the reference delegates to CBLAS (src/backend/cpu/blas.cpp:223-331,
   a third-party library with no pinned version) is numpy's ``matmul`` on
@@ -991,3 +996,299 @@ def canny(img: np.ndarray, threshold_type: int, t1: float, t2: float, sw: int = 
             for b2 in range(d2):
                 out[:, :, b2, b3] = _hysteresis(strong[:, :, b2, b3], weak[:, :, b2, b3])
     return out
+
+
+# -------------------------------------------------------------------- ORB
+ORB_PATCH = 31  # REF_PAT_SIZE, src/backend/cpu/kernel/orb.hpp:17
+
+
+def orb_pattern() -> np.ndarray:
+    """The 256 (x1, y1, x2, y2) pairs of the rotated-BRIEF pattern
+    (src/backend/cpu/kernel/orb.hpp:24-94), read from the product's data table
+    (a table, not code: its content is pinned by a digest of the reference's
+    table in tests/golden)."""
+    import re
+
+    path = os.path.join(_HERE, "..", "arrayfire_amd", "csrc", "kernels", "orb_pattern.inc")
+    body = "\n".join(l for l in open(path) if not l.lstrip().startswith("//"))
+    v = [int(t) for t in re.findall(r"-?\d+", body)]
+    return np.array(v, dtype=np.int32).reshape(256, 4)
+
+
+def resize_bilinear(img: np.ndarray, od0: int, od1: int) -> np.ndarray:
+    """resize_op<T, AF_INTERP_BILINEAR> (src/backend/cpu/kernel/resize.hpp:86-130)
+    for one 2-D slice: source coordinates and weights in float, the four
+    weighted taps summed left to right in the working type (float for f32,
+    double for f64, the weights rounded to float first)."""
+    id0, id1 = img.shape
+    f32 = np.float32
+    xs = np.arange(od0, dtype=np.float32) / (f32(od0) / f32(id0))
+    ys = np.arange(od1, dtype=np.float32) / (f32(od1) / f32(id1))
+    i1x = np.minimum(np.floor(xs).astype(np.int64), id0 - 1)
+    i1y = np.minimum(np.floor(ys).astype(np.int64), id1 - 1)
+    b = (xs - i1x.astype(np.float32)).astype(np.float32)
+    a = (ys - i1y.astype(np.float32)).astype(np.float32)
+    i2x = np.where(i1x + 1 >= id0, id0 - 1, i1x + 1)
+    i2y = np.where(i1y + 1 >= id1, id1 - 1, i1y + 1)
+    B, A = np.meshgrid(b, a, indexing="ij")
+    one = f32(1.0)
+    wt = img.dtype.type
+    w1 = ((one - A) * (one - B)).astype(wt)
+    w2 = (A * (one - B)).astype(wt)
+    w3 = ((one - A) * B).astype(wt)
+    w4 = (A * B).astype(wt)
+    p1 = img[np.ix_(i1x, i1y)]
+    p2 = img[np.ix_(i1x, i2y)]
+    p3 = img[np.ix_(i2x, i1y)]
+    p4 = img[np.ix_(i2x, i2y)]
+    return (((w1 * p1 + w2 * p2) + w3 * p3) + w4 * p4).astype(wt)
+
+
+# The 16 offsets (dy, dx) of FAST's circle, idx_y/idx_x of
+# src/backend/cpu/kernel/fast.hpp:20-31.
+def _fast_circle():
+    def iy(i):
+        return max(-3, min(3, -(i - 12) if i >= 8 else i - 4))
+
+    def ix(i):
+        return iy(i + 4) if i < 12 else iy(i - 12)
+    return [(iy(i), ix(i)) for i in range(16)]
+
+
+def fast(img: np.ndarray, thr: float, arc_length: int, nonmax: bool, feature_ratio: float, edge: int):
+    """cpu::fast (src/backend/cpu/fast.cpp:27-108) with locate_features and
+    non_maximal (src/backend/cpu/kernel/fast.hpp:51-214): a pixel is a corner
+    when arc_length contiguous circle pixels are all brighter than p + thr or
+    all darker than p - thr (all in float); its score is the larger of the
+    bright and dark sums of |p_x - p| - thr, summed in circle order. Corners
+    are found in scan order (dim0 index y outer, dim1 index x inner) and the
+    first ceil(elements * feature_ratio) are kept; suppression keeps those
+    above their 8 neighbours on the score map, with the reference's edge test
+    (which compares y with dims[1] and x with dims[0]). Returns float32
+    (x, y, score)."""
+    f32 = np.float32
+    d0, d1 = img.shape
+    max_feat = int(math.ceil(f32(img.size) * f32(feature_ratio)))
+    thr = f32(thr)
+    im = img.astype(np.float32)
+    ys = np.arange(edge, d0 - edge)
+    xs = np.arange(edge, d1 - edge)
+    e = np.zeros(0, np.float32)
+    if ys.size == 0 or xs.size == 0:
+        return e, e, e
+    Y, X = np.meshgrid(ys, xs, indexing="ij")  # scan order: y outer, x inner
+    p = im[Y, X]
+    circ = _fast_circle()
+    tests = []
+    for dy, dx in circ:
+        px = im[Y + dy, X + dx]
+        tests.append((px > p + thr).astype(np.int32) - (px < p - thr).astype(np.int32))
+    t = np.stack(tests)  # [16, ny, nx]
+    t2 = np.concatenate([t, t[:arc_length - 1]])
+    cs = np.concatenate([np.zeros((1,) + t.shape[1:], np.int64), np.cumsum(t2, axis=0)])
+    wins = cs[arc_length:] - cs[:-arc_length]  # every cyclic window of arc_length
+    corner = (wins.max(axis=0) == arc_length) | (wins.min(axis=0) == -arc_length)
+    sb = np.zeros(p.shape, np.float32)
+    sd = np.zeros(p.shape, np.float32)
+    for dy, dx in circ:
+        px = im[Y + dy, X + dx]
+        gt = (px > p + thr).astype(np.float32)
+        lt = (px < p - thr).astype(np.float32)
+        sb = sb + gt * (np.abs(px - p) - thr)
+        sd = sd + lt * (np.abs(p - px) - thr)
+    score = np.maximum(sb, sd)
+    fy, fx = Y[corner], X[corner]
+    fs = score[corner]
+    n = min(max_feat, fy.size)
+    fy, fx, fs = fy[:n], fx[:n], fs[:n]
+    if not nonmax:
+        return fx.astype(np.float32), fy.astype(np.float32), fs.astype(np.float32)
+    V = np.zeros((d0, d1), np.float32)
+    V[fy, fx] = fs
+    v = V[fy, fx]
+    mx = np.full(v.shape, -np.inf, np.float32)
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            if dy or dx:
+                mx = np.maximum(mx, V[fy + dy, fx + dx])
+    skip = (fy >= d1 - edge - 1) | (fy <= edge + 1) | (fx >= d0 - edge - 1) | (fx <= edge + 1)
+    keep = (~skip) & (v > mx)
+    return fx[keep].astype(np.float32), fy[keep].astype(np.float32), v[keep].astype(np.float32)
+
+
+def _orb_harris(img, xs, ys, block_size=7, k_thr=0.04, patch_size=ORB_PATCH):
+    """harris_response<T, false> (src/backend/cpu/kernel/orb.hpp:110-162):
+    keeps the features whose rounded position is patch_r =
+    ceil(size*sqrt(2)/2) inside the image; gradients are T differences
+    stored in float, the three products summed in float over the block in
+    row-major block order; score = (det - k*tr^2) * 0.001^4."""
+    f32 = np.float32
+    d0, d1 = img.shape
+    x = _round_half_away(xs.astype(np.float64)).astype(np.int64)
+    y = _round_half_away(ys.astype(np.float64)).astype(np.int64)
+    size = f32(2.0) * f32(math.floor(f32(patch_size) / f32(2.0))) + f32(1.0)
+    pr = int(math.ceil(f32(size) * f32(math.sqrt(2.0)) / f32(2.0)))
+    ok = ~((x < pr) | (y < pr) | (x >= d1 - pr) | (y >= d0 - pr))
+    x, y = x[ok], y[ok]
+    r = block_size // 2
+    ixx = np.zeros(x.shape, np.float32)
+    iyy = np.zeros(x.shape, np.float32)
+    ixy = np.zeros(x.shape, np.float32)
+    for k in range(block_size * block_size):
+        i = k // block_size - r
+        j = k % block_size - r
+        ix = (img[y + j, x + i + 1] - img[y + j, x + i - 1]).astype(np.float32)
+        iy = (img[y + j + 1, x + i] - img[y + j - 1, x + i]).astype(np.float32)
+        ixx = ixx + ix * ix
+        iyy = iyy + iy * iy
+        ixy = ixy + ix * iy
+    tr = ixx + iyy
+    det = ixx * iyy - ixy * ixy
+    resp = det - f32(k_thr) * (tr * tr)
+    rs = f32(0.001)
+    rs = rs * rs * rs * rs
+    return x.astype(np.float32), y.astype(np.float32), (resp * rs).astype(np.float32)
+
+
+def _orb_angle(img, x, y, patch_size=ORB_PATCH):
+    """centroid_angle (src/backend/cpu/kernel/orb.hpp:164-193): the patch's
+    first moments m01 = sum j*p, m10 = sum i*p accumulated in T in row-major
+    patch order, angle = atan2(m01, m10) (evaluated in double, stored as
+    float)."""
+    T = img.dtype.type
+    xi = x.astype(np.int64)
+    yi = y.astype(np.int64)
+    r = patch_size // 2
+    m01 = np.zeros(xi.shape, img.dtype)
+    m10 = np.zeros(xi.shape, img.dtype)
+    for k in range(patch_size * patch_size):
+        i = k // patch_size - r
+        j = k % patch_size - r
+        p = img[yi + j, xi + i]
+        m01 = m01 + T(j) * p
+        m10 = m10 + T(i) * p
+    return np.array([np.float32(math.atan2(float(a), float(b))) for a, b in zip(m01, m10)], np.float32)
+
+
+def _orb_describe(img, x, y, ori, scl, patch_size=ORB_PATCH):
+    """extract_orb and get_pixel (src/backend/cpu/kernel/orb.hpp:195-264):
+    each of the 256 pairs rotated by the feature's angle (sin and cos of the
+    float angle, rounded to float; offsets dist*scale*cos - dist*scale*sin in
+    float, rounded half away from zero), bit j of word i set when the first
+    sample is below the second. Returns (desc[n, 8] uint32, x, y, size) with
+    the positions scaled to the level's scale."""
+    f32 = np.float32
+    pat = orb_pattern()
+    d0, d1 = img.shape
+    n = x.size
+    desc = np.zeros((n, 8), np.uint32)
+    xi = x.astype(np.int64)
+    yi = y.astype(np.int64)
+    pscl = f32(patch_size) / f32(patch_size)
+    for f in range(n):
+        s = f32(math.sin(float(ori[f])))
+        c = f32(math.cos(float(ori[f])))
+
+        def px(dx, dy):
+            ox = _round_half_away((dx.astype(np.float32) * pscl * c - dy.astype(np.float32) * pscl * s)
+                                  .astype(np.float64))
+            oy = _round_half_away((dx.astype(np.float32) * pscl * s + dy.astype(np.float32) * pscl * c)
+                                  .astype(np.float64))
+            return img[(yi[f] + oy).astype(np.int64), (xi[f] + ox).astype(np.int64)]
+        p1 = px(pat[:, 0], pat[:, 1])
+        p2 = px(pat[:, 2], pat[:, 3])
+        bits = (p1 < p2).astype(np.uint64).reshape(8, 32)
+        desc[f] = (bits << np.arange(32, dtype=np.uint64)).sum(axis=1).astype(np.uint32)
+    sc = f32(scl)
+    ox = _round_half_away((x.astype(np.float32) * sc).astype(np.float64)).astype(np.float32)
+    oy = _round_half_away((y.astype(np.float32) * sc).astype(np.float64)).astype(np.float32)
+    size = np.full(n, f32(patch_size) * sc, np.float32)
+    return desc, ox, oy, size
+
+
+_LIBM = None
+
+
+def _powf(b, e) -> np.float32:
+    """std::pow(float, float), i.e. the C library's powf (numpy's float32
+    power rounds differently, e.g. 1.2f**3)."""
+    global _LIBM
+    if _LIBM is None:
+        _LIBM = ctypes.CDLL("libm.so.6")
+        _LIBM.powf.restype = ctypes.c_float
+        _LIBM.powf.argtypes = [ctypes.c_float, ctypes.c_float]
+    return np.float32(_LIBM.powf(float(np.float32(b)), float(np.float32(e))))
+
+
+def orb(img: np.ndarray, fast_thr: float, max_feat: int, scl_fctr: float, levels: int, blur_img: bool):
+    """af_orb (src/api/c/orb.cpp:58-89) with cpu::orb
+    (src/backend/cpu/orb.cpp:44-283): per pyramid level (each level resized
+    bilinearly from the previous one), FAST corners (threshold fast_thr, arc
+    9, suppression, feature ratio 0.15, edge 22), their Harris responses
+    (block 7, k 0.04), the lvl_best strongest by a stable descending sort,
+    the centroid angle on the level image, then the rotated-BRIEF descriptor
+    on the level image smoothed by the 9-tap sigma-2 Gaussian (when
+    blur_img). Levels are concatenated in order. Returns (x, y, score, ori,
+    size, desc[n, 8])."""
+    f32 = np.float32
+    x = np.asarray(img)
+    if x.ndim != 2 or x.shape[0] < 7 or x.shape[1] < 7:
+        raise ValueError(202)
+    if not (f32(fast_thr) > 0 and max_feat > 0 and f32(scl_fctr) > 1 and levels > 0):
+        raise ValueError(202)
+    if x.dtype not in (np.float32, np.float64):
+        raise ValueError(204)
+    sf = f32(scl_fctr)
+    patch = ORB_PATCH
+    min_side = f32(min(x.shape))
+    max_levels = 0
+    scl_sum = f32(0)
+    for i in range(levels):
+        min_side = f32(min_side / sf)
+        if min_side < patch or max_levels == levels:
+            break
+        max_levels += 1
+        scl_sum = f32(scl_sum + f32(1) / f32(_powf(sf, i)))
+    e = np.zeros(0, np.float32)
+    if max_levels == 0:
+        return e, e, e, e, e, np.zeros((0, 8), np.uint32)
+    lvl_best = []
+    feat_sum = 0
+    for i in range(max_levels - 1):
+        ls = f32(_powf(sf, i))
+        lvl_best.append(int(math.ceil(f32(f32(max_feat) / scl_sum) / ls)))
+        feat_sum += lvl_best[-1]
+    lvl_best.append((max_feat - feat_sum) % (1 << 32))
+    gauss = harris_filter(9, 2.0, x.dtype) if blur_img else None
+    size = f32(2) * f32(math.floor(f32(patch) / f32(2))) + f32(1)
+    edge = int(math.ceil(f32(size) * f32(math.sqrt(2.0)) / f32(2)))
+    prev = x
+    out = []
+    for i in range(max_levels):
+        ls = f32(_powf(sf, i))
+        if i == 0:
+            lvl = x
+        else:
+            ld0 = int(_round_half_away(np.float64(f32(f32(x.shape[0]) / ls))))
+            ld1 = int(_round_half_away(np.float64(f32(f32(x.shape[1]) / ls))))
+            lvl = resize_bilinear(prev, ld0, ld1)
+            prev = lvl
+        fx, fy, _ = fast(lvl, fast_thr, 9, True, 0.15, edge)
+        if fx.size == 0:
+            continue
+        hx, hy, hs = _orb_harris(lvl, fx, fy)
+        if hx.size == 0:
+            continue
+        order = np.argsort(-hs, kind="stable")
+        usable = min(hx.size, lvl_best[i])
+        if usable == 0:
+            continue
+        order = order[:usable]
+        lx, ly, lsc = hx[order], hy[order], hs[order]
+        lori = _orb_angle(lvl, lx, ly)
+        src = convolve2_sep(gauss, gauss, lvl, False)[:, :, 0, 0] if blur_img else lvl
+        desc, ox, oy, osz = _orb_describe(src, lx, ly, lori, ls)
+        out.append((ox, oy, lsc, lori, osz, desc))
+    if not out:
+        return e, e, e, e, e, np.zeros((0, 8), np.uint32)
+    return tuple(np.concatenate([o[k] for o in out]) for k in range(6))
