@@ -100,16 +100,23 @@ int autotune_pick(const std::string &key, int ncand, const std::function<void(in
     hipEvent_t e0 = ev0.e, e1 = ev1.e;
     int best = 0;
     float best_ms = 1e30f;
+    // Each candidate: one warm-up launch (code load, caches), then the
+    // fastest of three trials of three launches, so that one disturbed trial
+    // does not decide the choice.
     for (int c = 0; c < ncand; ++c) {
-        launch(c);  // warm-up (code load, caches)
-        check_hip(hipEventRecord(e0, s), "hipEventRecord");
-        for (int r = 0; r < 3; ++r) launch(c);
-        check_hip(hipEventRecord(e1, s), "hipEventRecord");
-        check_hip(hipEventSynchronize(e1), "hipEventSynchronize");
-        float ms = 0.f;
-        check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-        if (ms < best_ms) {
-            best_ms = ms;
+        launch(c);
+        float cand_ms = 1e30f;
+        for (int trial = 0; trial < 3; ++trial) {
+            check_hip(hipEventRecord(e0, s), "hipEventRecord");
+            for (int r = 0; r < 3; ++r) launch(c);
+            check_hip(hipEventRecord(e1, s), "hipEventRecord");
+            check_hip(hipEventSynchronize(e1), "hipEventSynchronize");
+            float ms = 0.f;
+            check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+            if (ms < cand_ms) cand_ms = ms;
+        }
+        if (cand_ms < best_ms) {
+            best_ms = cand_ms;
             best = c;
         }
     }
