@@ -241,7 +241,10 @@ static std::string nn_key(const char *tag, int pass, const ArrayImpl &in, af_dty
                      : in.type == f32                     ? (int)AFHIP_PRECISION_F32
                      : in.type == afhip_bf16              ? (int)AFHIP_PRECISION_BF16
                                                           : (int)AFHIP_PRECISION_F16;
-    std::string k = std::string(tag) + ":p" + std::to_string(pass) + ":m" + std::to_string(mode);
+    // the storage type too: f32 arrays at a 16-bit precision run other
+    // kernels (element gather) than 16-bit arrays (channel-last copies)
+    std::string k = std::string(tag) + ":p" + std::to_string(pass) + ":m" + std::to_string(mode) + ":t" +
+                    std::to_string((int)in.type);
     for (int i = 0; i < 4; ++i) k += (i ? "," : ":x") + std::to_string(g.x[i]);
     for (int i = 0; i < 4; ++i) k += (i ? "," : ":w") + std::to_string(g.w[i]);
     k += ":s" + std::to_string(g.st[0]) + "," + std::to_string(g.st[1]) + ":pd" + std::to_string(g.pd[0]) + "," +

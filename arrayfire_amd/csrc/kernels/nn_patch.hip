@@ -908,17 +908,20 @@ __global__ __launch_bounds__(Q_THREADS, 2) void conv_pipe(TOut *__restrict__ y,
 }
 
 // ---------------------------------------------------------------- weights-resident kernel
-// conv_wres: the forward pass for GEMMs whose reduction (NCH channel chunks
-// of 32 x 9 taps) fits in registers, 16-bit operands.
-// Unit of work: (tile of 32*NQ filters, image, band of RB output rows); one
-// persistent workgroup of 8 waves per CU walks units lb, lb + G, ... in
-// filter-tile order. Wave w: wq = w % NQ picks 32 filters of the tile, pg =
-// w / NQ one of PG = 8 / NQ pixel groups. Each wave keeps the B fragments
-// of its 32 filters for every k-step in VGPRs (NCH*9 x 2 fragments of 4
-// VGPRs), reloaded only when the unit's filter tile changes, so the MFMA
-// loop reads nothing but the patch from LDS: per k-step 2 A fragments (16
-// pixels x 32 channels at the tap's offset, read a k-step ahead) feed 4
-// MFMAs, with no barrier. The band's patch (every channel chunk; channel-last
+// conv_wres: the forward pass, or the stride-1 data gradient (the same
+// convolution of dy with the unflipped, transposed filters), for GEMMs whose
+// reduction (NCH channel chunks of 32 x 9 taps) fits in registers, 16-bit
+// operands.
+// Unit of work: (tile of 16*NF*NQ filters, image, band of RB output rows);
+// one persistent workgroup of 8 waves per CU walks units lb, lb + G, ... in
+// filter-tile order. Wave w: wq = w % NQ picks 16*NF filters of the tile,
+// pg = w / NQ one of PG = 8 / NQ pixel groups. Each wave keeps the B
+// fragments of its filters for every k-step in VGPRs (NCH*9 x NF fragments
+// of 4 VGPRs: NF = 2 for up to 64 channels, NF = 1 for up to 128), reloaded
+// only when the unit's filter tile changes, so the MFMA loop reads nothing
+// but the patch from LDS: per k-step 2 A fragments (16 pixels x 32 channels
+// at the tap's offset, read a k-step ahead) feed 2*NF MFMAs, with no
+// barrier. The band's patch (every channel chunk; channel-last
 // 64-byte pixel rows swizzled as q_swz, the segment transposes of conv_pipe)
 // arrives in two row pieces: the loads of both are issued at once, the first
 // piece is stored, and the output rows it covers are computed while the
@@ -940,11 +943,12 @@ struct WresParams {
     int vec;                  // OW % 4 == 0 and one output image under 2^31 - 64 bytes: buffer stores
 };
 
-template <int MODE, typename TIn, typename TOut, int NCH, int NQ, int PASS>
+template <int MODE, typename TIn, typename TOut, int NCH, int NQ, int NF, int PASS>
 __global__ __launch_bounds__(WR_THREADS, 1) void conv_wres(TOut *__restrict__ y, const TIn *__restrict__ x,
                                                            const char *__restrict__ wpk, WresParams p) {
     static_assert(MODE == PM_BF16 || MODE == PM_F16, "16-bit operands");
-    constexpr int RS = 9, NKS = NCH * RS, SEG = 8, KC = 32, NF = 2, PG = WR_WAVES / NQ;
+    static_assert(NCH * NF <= 4, "resident weights: at most 144 VGPRs");
+    constexpr int RS = 9, NKS = NCH * RS, SEG = 8, KC = 32, PG = WR_WAVES / NQ, FPW = 16 * NF;
     constexpr int kOOB = 0x7fffffc0;
     extern __shared__ __attribute__((aligned(16))) char smem[];
     const int tid = threadIdx.x, lane = tid & 63;
@@ -967,9 +971,9 @@ __global__ __launch_bounds__(WR_THREADS, 1) void conv_wres(TOut *__restrict__ y,
     const auto wr = __builtin_amdgcn_make_buffer_rsrc(const_cast<char *>(wpk), (short)0, p.w_bytes, 0x00020000);
     auto load_weights = [&](int kt) __attribute__((always_inline)) {
         // packed [kt128][chunk][tap][128 rows][32 ch]; this wave's filters
-        // f0 + fi * 16 + lane % 16 with f0 = kt * 32 * NQ + wq * 32 stay in one
-        // 128-row panel
-        const int f0 = kt * 32 * NQ + wq * 32;
+        // f0 + fi * 16 + lane % 16 with f0 = kt * FPW * NQ + wq * FPW stay in
+        // one 128-row panel
+        const int f0 = kt * FPW * NQ + wq * FPW;
         const int k128 = f0 >> 7, row = (f0 & 127) + (lane & 15);
 #pragma unroll
         for (int ks = 0; ks < NKS; ++ks)
@@ -1115,7 +1119,7 @@ __global__ __launch_bounds__(WR_THREADS, 1) void conv_wres(TOut *__restrict__ y,
         int kfo[NF];
 #pragma unroll
         for (int fi = 0; fi < NF; ++fi) {
-            const int kf = kt * 32 * NQ + wq * 32 + fi * 16 + (lane & 15);
+            const int kf = kt * FPW * NQ + wq * FPW + fi * 16 + (lane & 15);
             kfo[fi] = kf < p.K ? kf * P + r0 * p.OW : -1;
         }
         TOut *yn = y + (long long)n * p.K * P;
@@ -1375,19 +1379,25 @@ static int launch_pipe(void *y, const void *x, const void *ws, const PatchParams
 // first ceil(RB / 2) output rows. AFHIP_NN_WRES=0 in the environment
 // disables the kernel.
 template <int MODE, typename TIn>
-static bool plan_wres(WresParams &w, int &nch, int &nq, const PatchParams &p, const void *x, int dgrad,
+static bool plan_wres(WresParams &w, int &nch, int &nq, int &nf, const PatchParams &p, const void *x, int dgrad,
                       long long wsz) {
     if constexpr (MODE == PM_SPLIT || MODE == PM_F32 || sizeof(TIn) != 2) {
         return false;
     } else {
         const char *off = std::getenv("AFHIP_NN_WRES");
         if (off && off[0] == '0') return false;
-        if (dgrad) return false;
-        if (p.fw != 3 || p.fh != 3 || p.d0 != 1 || p.C > 64 || p.C <= 0) return false;
+        if (dgrad) {
+            const char *offd = std::getenv("AFHIP_NN_WRES_DGRAD");
+            if (offd && offd[0] == '0') return false;
+        }
+        // forward: up to 64 channels (32 filters per wave); the data gradient
+        // also up to 128 (16 filters per wave)
+        if (p.fw != 3 || p.fh != 3 || p.d0 != 1 || p.C <= 0 || p.C > (dgrad ? 128 : 64)) return false;
         if (p.W % 8 != 0 || (reinterpret_cast<uintptr_t>(x) & 15) != 0) return false;
         if (p.p0 < 0 || p.p1 < 0) return false;
         nch = (p.C + 31) / 32;
-        nq = p.K > 64 ? 4 : 2;
+        nf = nch <= 2 ? 2 : 1;
+        nq = p.K > 32 * nf ? 4 : 2;
         w.W = p.W; w.H = p.H; w.C = p.C; w.N = p.N; w.OW = p.OW; w.OH = p.OH; w.K = p.K;
         w.s0 = p.s0; w.s1 = p.s1; w.p0 = p.p0; w.p1 = p.p1; w.d1 = p.d1;
         w.nseg = (p.W + 15) / 16;
@@ -1397,7 +1407,7 @@ static bool plan_wres(WresParams &w, int &nch, int &nq, const PatchParams &p, co
         // block) are not stored
         const int need = max((p.OW - 1) * p.s0 + 2 * p.d0 + 1, p.p0 + p.W);
         w.PW = (need + 7) & ~7;
-        w.ntw = (p.K + 32 * nq - 1) / (32 * nq);
+        w.ntw = (p.K + 16 * nf * nq - 1) / (16 * nf * nq);
         const long long cus = device_cu_count();
         long long best = -1;
         for (int rb = 1; rb <= p.OH; ++rb) {
@@ -1427,14 +1437,21 @@ static bool plan_wres(WresParams &w, int &nch, int &nq, const PatchParams &p, co
     }
 }
 
-template <int MODE, typename TIn, typename TOut>
-static int launch_wres(void *y, const void *x, const void *ws, const WresParams &w, int nch, int nq,
+template <int MODE, typename TIn, typename TOut, int PASS>
+static int launch_wres(void *y, const void *x, const void *ws, const WresParams &w, int nch, int nq, int nf,
                        hipStream_t s) {
     using Fn = void (*)(TOut *, const TIn *, const char *, WresParams);
-    const Fn fn = nch == 1 ? (nq == 4 ? conv_wres<MODE, TIn, TOut, 1, 4, 0> : conv_wres<MODE, TIn, TOut, 1, 2, 0>)
-                           : (nq == 4 ? conv_wres<MODE, TIn, TOut, 2, 4, 0> : conv_wres<MODE, TIn, TOut, 2, 2, 0>);
+    Fn fn;
+    if (nf == 2)
+        fn = nch == 1 ? (nq == 4 ? conv_wres<MODE, TIn, TOut, 1, 4, 2, PASS> : conv_wres<MODE, TIn, TOut, 1, 2, 2, PASS>)
+                      : (nq == 4 ? conv_wres<MODE, TIn, TOut, 2, 4, 2, PASS> : conv_wres<MODE, TIn, TOut, 2, 2, 2, PASS>);
+    else if constexpr (PASS == 1)
+        fn = nch == 3 ? (nq == 4 ? conv_wres<MODE, TIn, TOut, 3, 4, 1, PASS> : conv_wres<MODE, TIn, TOut, 3, 2, 1, PASS>)
+                      : (nq == 4 ? conv_wres<MODE, TIn, TOut, 4, 4, 1, PASS> : conv_wres<MODE, TIn, TOut, 4, 2, 1, PASS>);
+    else
+        return K_ERR_ARG;
     const size_t lds = (size_t)nch * w.PR * w.PW * Q_PB;
-    static bool attr_set[4] = {false, false, false, false};
+    static bool attr_set[8] = {false, false, false, false, false, false, false, false};
     const int slot = (nch - 1) * 2 + (nq == 4);
     if (!attr_set[slot]) {
         if (hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1473,10 +1490,11 @@ static int launch(void *y, const void *x, const void *w, PatchParams &p, int dgr
         if constexpr (MODE != PM_SPLIT && sizeof(TIn) == PCfg<MODE>::ESZ) {
             if constexpr (MODE == PM_BF16 || MODE == PM_F16) {
                 WresParams wr;
-                int nch = 0, nq = 0;
+                int nch = 0, nq = 0, nf = 0;
                 const long long wsz = (long long)p.ktiles * p.nchunks * p.RS * P_BM * PCfg<MODE>::KC * 2;
-                if (plan_wres<MODE, TIn>(wr, nch, nq, p, x, dgrad, wsz))
-                    return launch_wres<MODE, TIn, TOut>(y, x, ws, wr, nch, nq, s);
+                if (plan_wres<MODE, TIn>(wr, nch, nq, nf, p, x, dgrad, wsz))
+                    return dgrad ? launch_wres<MODE, TIn, TOut, 1>(y, x, ws, wr, nch, nq, nf, s)
+                                 : launch_wres<MODE, TIn, TOut, 0>(y, x, ws, wr, nch, nq, nf, s);
             }
             if (plan_pipe<MODE, TIn>(q, p, x, wm)) return launch_pipe<MODE, TIn, TOut>(y, x, ws, q, wm, dgrad, s);
         }
@@ -1577,9 +1595,9 @@ extern "C" long long afhip_k_conv2_nn_patch_tiles(int pass, int in_type, int pre
         if constexpr (MD != PM_SPLIT && sizeof(TIn) == PCfg<MD>::ESZ) {
             if constexpr (MD == PM_BF16 || MD == PM_F16) {
                 WresParams wr;
-                int nch = 0, nq = 0;
+                int nch = 0, nq = 0, nf = 0;
                 const long long wsz = (long long)p.ktiles * p.nchunks * p.RS * P_BM * PCfg<MD>::KC * 2;
-                if (plan_wres<MD, TIn>(wr, nch, nq, p, nullptr, pass, wsz)) return wr.nunits;
+                if (plan_wres<MD, TIn>(wr, nch, nq, nf, p, nullptr, pass, wsz)) return wr.nunits;
             }
             PatchParams q;
             int wm = 0;

@@ -81,6 +81,7 @@ def test_pipe_forward(af, rng, g, dt, monkeypatch):
 @pytest.mark.parametrize("dt", ["bf16", "f32"])
 @pytest.mark.parametrize("g", PIPE_GEOMS, ids=_id)
 def test_pipe_data_gradient(af, rng, g, dt, monkeypatch):
+    monkeypatch.setenv("AFHIP_NN_WRES_DGRAD", "0")  # the pipelined kernel, not the weights-resident one
     X, W = _arrays(af, rng, g, dt)
     y = af.convolve2NN(X, W, g[2], g[3], g[4], out_type=af.f32)
     dy = rng.standard_normal(y.dims).astype(np.float32)

@@ -1,7 +1,9 @@
-"""The weights-resident convolve2_nn forward kernel (conv_wres: the 32-filter
-weight fragments of every k-step held in registers, the band's patch gathered
-once per unit) on the geometries it serves: 16-bit operands stored as such,
-3x3 filters with dilation 1 along W, C <= 64, W % 8 == 0.
+"""The weights-resident convolve2_nn kernel (conv_wres: the weight fragments
+of every k-step held in registers, the band's patch gathered once per unit)
+on the geometries it serves: 16-bit operands stored as such, 3x3 filters
+with dilation 1 along W; the forward for C <= 64 and W % 8 == 0 (32 filters
+per wave), the stride-1 data gradient for K <= 128 and OW % 8 == 0 (32
+filters per wave up to 64 of dy's channels, 16 above).
 
 Cases: the benchmarked layer's shape on fewer images, one and two channel
 chunks with a tail (C = 16, 48), filters past one 64-filter tile and not a
@@ -76,3 +78,46 @@ def test_wres_forward(af, g, dt, monkeypatch):
     y16 = af.convolve2NN(X, W, g[2], g[3], g[4])
     assert y16.type == t
     assert oracle.rel_l2(_rounded(y16), ref) <= 4e-3
+
+
+# Data gradient: (x dims, w dims, stride 1, pad, dilation); dy has K channels
+# (the reduction: 2, 3 or 4 chunks of 32) and OW % 8 == 0.
+WRES_DGRAD_GEOMS = [
+    ((56, 56, 64, 4), (3, 3, 64, 128), (1, 1), (1, 1), (1, 1)),
+    ((56, 56, 128, 2), (3, 3, 128, 64), (1, 1), (1, 1), (1, 1)),
+    ((24, 20, 70, 3), (3, 3, 70, 96), (1, 1), (1, 1), (1, 1)),
+    ((16, 16, 24, 300), (3, 3, 24, 100), (1, 1), (1, 1), (1, 1)),
+    ((18, 18, 32, 2), (3, 3, 32, 128), (1, 1), (0, 0), (1, 1)),
+    ((32, 30, 64, 2), (3, 3, 64, 96), (1, 1), (1, 2), (1, 2)),
+    ((200, 20, 96, 1), (3, 3, 96, 64), (1, 1), (1, 1), (1, 1)),
+]
+
+
+@pytest.mark.parametrize("dt", ["bf16", "f16"])
+@pytest.mark.parametrize("g", WRES_DGRAD_GEOMS, ids=_id)
+def test_wres_data_gradient(af, g, dt, monkeypatch):
+    rng = np.random.default_rng(abs(hash(("dgrad",) + g)) % 2**32)
+    x = rng.standard_normal(g[0]).astype(np.float32)
+    fan = g[1][0] * g[1][1] * g[1][2]
+    w = (rng.standard_normal(g[1]) * np.sqrt(2.0 / fan)).astype(np.float32)
+    t = af.bf16 if dt == "bf16" else af.f16
+    X, W = af.Array(x).astype(t), af.Array(w).astype(t)
+    y = af.convolve2NN(X, W, g[2], g[3], g[4], out_type=af.f32)
+    assert y.dims[0] % 8 == 0
+    dy = rng.standard_normal(y.dims).astype(np.float32)
+    DY = af.Array(dy).astype(t)
+    dx = af.convolve2GradientNN(DY, X, W, y, *g[2:], af.AF_CONV_GRADIENT_DATA, out_type=af.f32)
+    ref = oracle.conv2_data_grad_direct_f64(_rounded(DY), _rounded(X), _rounded(W), *g[2:])
+    assert dx.dims == X.dims
+    dxn = dx.to_numpy()
+    assert oracle.rel_l2(dxn, ref) <= 1e-5
+    monkeypatch.setenv("AFHIP_NN_WRES_DGRAD", "0")
+    dx0 = af.convolve2GradientNN(DY, X, W, y, *g[2:], af.AF_CONV_GRADIENT_DATA, out_type=af.f32).to_numpy()
+    e = oracle.rel_l2(dxn, dx0)
+    print(f"dgrad {_id(g)} {dt}: vs pipelined {e:.2e} exact={np.array_equal(dxn, dx0)}", flush=True)
+    assert e <= 1e-6
+    monkeypatch.delenv("AFHIP_NN_WRES_DGRAD")
+    y16 = af.convolve2NN(X, W, g[2], g[3], g[4])
+    dx16 = af.convolve2GradientNN(DY, X, W, y16, *g[2:], af.AF_CONV_GRADIENT_DATA)
+    assert dx16.type == t
+    assert oracle.rel_l2(_rounded(dx16), ref) <= 4e-3
