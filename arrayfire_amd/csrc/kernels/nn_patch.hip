@@ -1390,9 +1390,8 @@ static bool plan_wres(WresParams &w, int &nch, int &nq, int &nf, const PatchPara
             const char *offd = std::getenv("AFHIP_NN_WRES_DGRAD");
             if (offd && offd[0] == '0') return false;
         }
-        // forward: up to 64 channels (32 filters per wave); the data gradient
-        // also up to 128 (16 filters per wave)
-        if (p.fw != 3 || p.fh != 3 || p.d0 != 1 || p.C <= 0 || p.C > (dgrad ? 128 : 64)) return false;
+        // up to 64 channels with 32 filters per wave, up to 128 with 16
+        if (p.fw != 3 || p.fh != 3 || p.d0 != 1 || p.C <= 0 || p.C > 128) return false;
         if (p.W % 8 != 0 || (reinterpret_cast<uintptr_t>(x) & 15) != 0) return false;
         if (p.p0 < 0 || p.p1 < 0) return false;
         nch = (p.C + 31) / 32;
@@ -1445,11 +1444,9 @@ static int launch_wres(void *y, const void *x, const void *ws, const WresParams 
     if (nf == 2)
         fn = nch == 1 ? (nq == 4 ? conv_wres<MODE, TIn, TOut, 1, 4, 2, PASS> : conv_wres<MODE, TIn, TOut, 1, 2, 2, PASS>)
                       : (nq == 4 ? conv_wres<MODE, TIn, TOut, 2, 4, 2, PASS> : conv_wres<MODE, TIn, TOut, 2, 2, 2, PASS>);
-    else if constexpr (PASS == 1)
+    else
         fn = nch == 3 ? (nq == 4 ? conv_wres<MODE, TIn, TOut, 3, 4, 1, PASS> : conv_wres<MODE, TIn, TOut, 3, 2, 1, PASS>)
                       : (nq == 4 ? conv_wres<MODE, TIn, TOut, 4, 4, 1, PASS> : conv_wres<MODE, TIn, TOut, 4, 2, 1, PASS>);
-    else
-        return K_ERR_ARG;
     const size_t lds = (size_t)nch * w.PR * w.PW * Q_PB;
     static bool attr_set[8] = {false, false, false, false, false, false, false, false};
     const int slot = (nch - 1) * 2 + (nq == 4);

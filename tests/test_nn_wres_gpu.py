@@ -1,9 +1,9 @@
 """The weights-resident convolve2_nn kernel (conv_wres: the weight fragments
 of every k-step held in registers, the band's patch gathered once per unit)
 on the geometries it serves: 16-bit operands stored as such, 3x3 filters
-with dilation 1 along W; the forward for C <= 64 and W % 8 == 0 (32 filters
-per wave), the stride-1 data gradient for K <= 128 and OW % 8 == 0 (32
-filters per wave up to 64 of dy's channels, 16 above).
+with dilation 1 along W, up to 128 reduction channels (32 filters per wave
+up to 64 channels, 16 above); the forward for W % 8 == 0, the stride-1 data
+gradient (dy's K channels reduced) for OW % 8 == 0.
 
 Cases: the benchmarked layer's shape on fewer images, one and two channel
 chunks with a tail (C = 16, 48), filters past one 64-filter tile and not a
@@ -41,6 +41,10 @@ WRES_GEOMS = [
     ((48, 12, 64, 4), (3, 3, 64, 200), (1, 2), (1, 0), (1, 1)),
     ((200, 50, 64, 1), (3, 3, 64, 64), (1, 1), (1, 1), (1, 1)),
     ((16, 16, 32, 300), (3, 3, 32, 128), (1, 1), (1, 1), (1, 1)),
+    # 16 filters per wave: 3 and 4 channel chunks
+    ((32, 32, 128, 4), (3, 3, 128, 128), (2, 2), (1, 1), (1, 1)),
+    ((56, 30, 96, 2), (3, 3, 96, 64), (1, 1), (1, 1), (1, 1)),
+    ((24, 20, 100, 3), (3, 3, 100, 70), (1, 1), (0, 2), (1, 2)),
 ]
 
 
