@@ -37,8 +37,8 @@ PEAK = {"hbm_GBs": 8000.0, "bf16_TFs": 2500.0, "fp32_TFs": 157.3,
         "fp32_valu_TFs": 78.6}
 # Mangled names of the step's kernels (as rocprofv3 reports them).
 KERNEL_SYMBOL = {
-    "conv_fwd": "afhip::conv_wres<1, unsigned short, unsigned short, 2, 4, 0>",
-    "conv_dgrad": "afhip::conv_pipe<1, unsigned short, unsigned short, 1, 3, 1, 3>",
+    "conv_fwd": "afhip::conv_wres<1, unsigned short, unsigned short, 2, 4, 2, 0>",
+    "conv_dgrad": "afhip::conv_wres<1, unsigned short, unsigned short, 4, 4, 1, 1>",
     "wgrad_partial": "afhip::wgrad_pc<3, 3, 1, 0, true>",
     "wgrad_reduce": "afhip::wgrad_sum<unsigned short>",
 }
