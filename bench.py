@@ -482,6 +482,68 @@ def spatial_secondary(af, ev, stream, reps=10, cpu=True):
     return res
 
 
+def callers_secondary(af, reps=5, cpu=True):
+    """The frequency-domain route and the callers of the convolution path
+    (DoG, IIR, Harris, Canny, ORB), each on one 1024x1024 f32 image through
+    the af_* API; wall clock around reps calls (Harris, Canny and ORB read
+    counts back to the host inside the call). GB/s counts the image read once
+    and the result written once, a lower bound on each pipeline's traffic, so
+    hbm_frac is an upper bound on how close it runs to HBM. The oracle's
+    restatement on the host beside each, on a crop (one thread, and a
+    thread pool)."""
+    import oracle
+    import scipy.ndimage as ndi
+
+    rng = np.random.default_rng(20261202 + 7)
+    base = ndi.gaussian_filter(rng.random((1024, 1024)), 2.0)
+    img = ((base - base.min()) / (base.max() - base.min()) * 255.0).astype(np.float32)
+    img += (rng.random(img.shape) > 0.997).astype(np.float32) * 60.0
+    X = af.Array(img)
+    nt = _cpu_threads()
+    res = {}
+
+    def wall(fn):
+        hold = fn()
+        af.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            hold = fn()
+        af.sync()
+        del hold
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    def entry(name, fn, out_bytes, cpu_fn, side, what):
+        ms = wall(fn)
+        nbytes = img.nbytes + out_bytes
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        d = {"ms": round(ms, 4), "GPix_s": round(img.size / (ms * 1e-3) / 1e9, 3), "GB_s": round(gbs, 1),
+             "hbm_frac": round(gbs / PEAK["hbm_GBs"], 4)}
+        if cpu:
+            crop = np.ascontiguousarray(img[:side, :side])
+            d["cpu_baseline"] = cpu_line(lambda c: cpu_fn(c), [crop] * nt, side * side / 1e9, "GPix/s",
+                                         f"{side}x{side} crop of the image, {what}")
+        res[name] = d
+
+    f31 = rng.standard_normal((31, 31)).astype(np.float32)
+    F31 = af.Array(f31)
+    entry("fft_convolve2_1024sq_31x31", lambda: af.convolve2(X, F31, af.AF_CONV_DEFAULT, af.AF_CONV_FREQ),
+          img.nbytes, lambda c: oracle.fft_convolve(c, f31, 2, False), 256, "the oracle's padded FFT product")
+    entry("dog_1024sq_r1_r2", lambda: af.dog(X, 1, 2), img.nbytes, lambda c: oracle.dog(c, 1, 2), 256,
+          "the oracle's two smoothings and difference")
+    b = np.array([0.2, 0.3, 0.2], np.float32)
+    a = np.array([1.0, -0.5, 0.1], np.float32)
+    B, A = af.Array(b), af.Array(a)
+    entry("iir_1024_columns_x1024", lambda: af.iir(B, A, X), img.nbytes, lambda c: oracle.iir(b, a, c), 256,
+          "the oracle's FIR part and recurrence per column")
+    entry("harris_1024sq_s1", lambda: af.harris(X, 500, 1e5, 1.0, 0, 0.04), 5 * 4 * 500,
+          lambda c: oracle.harris(c, 500, 1e5, 1.0, 0, 0.04), 128, "the oracle's restatement")
+    entry("canny_1024sq_manual", lambda: af.canny(X, 0, 0.1, 0.3), img.size, lambda c: oracle.canny(c, 0, 0.1, 0.3),
+          256, "the oracle's restatement")
+    entry("orb_1024sq_l4_blur", lambda: af.orb(X, 20.0, 400, 1.5, 4, True), 400 * (5 * 4 + 32),
+          lambda c: oracle.orb(c, 20.0, 400, 1.5, 4, True), 128, "the oracle's restatement")
+    return res
+
+
 def ragged_secondary(af, ev, stream, cpu=True):
     """cfg 5: 2000 independent spatial 2-D convolutions (committed list in
     tests/golden/ragged_problems.json) as one graph-captured persistent
@@ -777,6 +839,7 @@ def main():
     if not args.no_spatial:
         out["secondary_fp32_spatial"] = spatial_secondary(af, ev, stream, cpu=not args.no_cpu)
         out["secondary_fp32_ragged"] = ragged_secondary(af, ev, stream, cpu=not args.no_cpu)
+        out["secondary_callers"] = callers_secondary(af, cpu=not args.no_cpu)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))
