@@ -39,8 +39,8 @@ static af_array canny(const ArrayImpl &in, af_canny_threshold ct, float t1, floa
     check_k(afhip_k_canny_nms((float *)sup->ptr, (const float *)mag->ptr, (const float *)gx->ptr,
                               (const float *)gy->ptr, d0, d1, ns, s),
             "canny");
-    auto mm = scratch(2 * sizeof(float));
-    check_k(afhip_k_canny_minmax((float *)mm->ptr, (const float *)sup->ptr, n, s), "canny");
+    auto mm = scratch(2 * sizeof(float)), part = scratch(2048 * sizeof(float));
+    check_k(afhip_k_canny_minmax((float *)mm->ptr, (const float *)sup->ptr, n, (float *)part->ptr, s), "canny");
     std::shared_ptr<Buffer> thr;
     int mode = 0;
     if (ct == AF_CANNY_THRESHOLD_AUTO_OTSU) {

@@ -6,8 +6,8 @@
 //    borders; dx along dim0, dy along dim1) and the L1 or L2 magnitude;
 //  * canny_nms: interior pixels keep their magnitude when it exceeds both
 //    magnitudes interpolated along the gradient direction, others are 0;
-//  * canny_minmax: the suppressed map's min and max (MANUAL normalisation,
-//    AUTO_OTSU bin count);
+//  * canny_minmax_part + canny_minmax_final: the suppressed map's min and
+//    max (MANUAL normalisation, AUTO_OTSU bin count);
 //  * canny_hist + canny_otsu: per slice, the histogram over [0, max] and
 //    Otsu's threshold (bin index of the first largest between-class
 //    variance), its scans and sums in float in bin order on one lane;
@@ -117,16 +117,14 @@ __global__ __launch_bounds__(256) void canny_nms(float *__restrict__ out, const 
     }
 }
 
-// mm[0] = min, mm[1] = max over n values: one workgroup, fixed lane order
-// (min and max are exact, so any order gives the same values).
-__global__ __launch_bounds__(256) void canny_minmax(float *__restrict__ mm, const float *__restrict__ v,
-                                                    long long n) {
+// mm[0] = min, mm[1] = max over n values in two launches: each of up to
+// kMinMaxBlocks workgroups reduces a grid-stride share into part[2b],
+// part[2b+1], then one workgroup reduces those (min and max are exact, so
+// the grouping does not change the values).
+constexpr int kMinMaxBlocks = 1024;
+
+__device__ inline void block_minmax(float &lo, float &hi) {
     __shared__ float smin[256], smax[256];
-    float lo = INFINITY, hi = -INFINITY;
-    for (long long e = threadIdx.x; e < n; e += 256) {
-        lo = fminf(lo, v[e]);
-        hi = fmaxf(hi, v[e]);
-    }
     smin[threadIdx.x] = lo;
     smax[threadIdx.x] = hi;
     __syncthreads();
@@ -137,9 +135,35 @@ __global__ __launch_bounds__(256) void canny_minmax(float *__restrict__ mm, cons
         }
         __syncthreads();
     }
+    lo = smin[0];
+    hi = smax[0];
+}
+
+__global__ __launch_bounds__(256) void canny_minmax_part(float *__restrict__ part, const float *__restrict__ v,
+                                                         long long n) {
+    float lo = INFINITY, hi = -INFINITY;
+    for (long long e = blockIdx.x * 256LL + threadIdx.x; e < n; e += (long long)gridDim.x * 256) {
+        lo = fminf(lo, v[e]);
+        hi = fmaxf(hi, v[e]);
+    }
+    block_minmax(lo, hi);
     if (threadIdx.x == 0) {
-        mm[0] = smin[0];
-        mm[1] = smax[0];
+        part[2 * blockIdx.x] = lo;
+        part[2 * blockIdx.x + 1] = hi;
+    }
+}
+
+__global__ __launch_bounds__(256) void canny_minmax_final(float *__restrict__ mm, const float *__restrict__ part,
+                                                          int nb) {
+    float lo = INFINITY, hi = -INFINITY;
+    for (int b = threadIdx.x; b < nb; b += 256) {
+        lo = fminf(lo, part[2 * b]);
+        hi = fmaxf(hi, part[2 * b + 1]);
+    }
+    block_minmax(lo, hi);
+    if (threadIdx.x == 0) {
+        mm[0] = lo;
+        mm[1] = hi;
     }
 }
 
@@ -280,8 +304,12 @@ extern "C" int afhip_k_canny_nms(float *out, const float *mag, const float *gx, 
     return launch_status();
 }
 
-extern "C" int afhip_k_canny_minmax(float *mm, const float *v, long long n, void *stream) {
-    canny_minmax<<<1, 256, 0, (hipStream_t)stream>>>(mm, v, n);
+extern "C" int afhip_k_canny_minmax(float *mm, const float *v, long long n, float *part, void *stream) {
+    if (n < 1 || part == nullptr) return K_ERR_ARG;
+    long long nb = (n + 256LL * 16 - 1) / (256LL * 16);  // about 16 values per lane
+    if (nb > kMinMaxBlocks) nb = kMinMaxBlocks;
+    canny_minmax_part<<<(unsigned)nb, 256, 0, (hipStream_t)stream>>>(part, v, n);
+    canny_minmax_final<<<1, 256, 0, (hipStream_t)stream>>>(mm, part, (int)nb);
     return launch_status();
 }
 

@@ -157,12 +157,15 @@ __global__ __launch_bounds__(256) void orb_fast_nonmax(unsigned char *__restrict
     }
 }
 
-template <typename T>
+// BLOCK is the Harris window (7 in ORB); the loop is unrolled so that a
+// lane's loads are all in flight before the in-order sums.
+template <typename T, int BLOCK>
 __global__ __launch_bounds__(256) void orb_harris(unsigned char *__restrict__ flags, float *__restrict__ resp,
                                                   const T *__restrict__ img, const unsigned *__restrict__ sel1,
                                                   const unsigned *__restrict__ sel2, long long n, int d0, int d1,
-                                                  int edge, int nx, int patch_r, int block, float k_thr) {
+                                                  int edge, int nx, int patch_r, float k_thr) {
 #pragma clang fp contract(off)
+    constexpr int block = BLOCK;
     for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < n;
          j += (long long)gridDim.x * blockDim.x) {
         const unsigned k = sel1[sel2[j]];
@@ -172,8 +175,9 @@ __global__ __launch_bounds__(256) void orb_harris(unsigned char *__restrict__ fl
             resp[j] = 0.f;
             continue;
         }
-        const int r = block / 2;
+        constexpr int r = block / 2;
         float ixx = 0.f, iyy = 0.f, ixy = 0.f;
+#pragma unroll
         for (int q = 0; q < block * block; ++q) {
             const int i = q / block - r, jj = q % block - r;
             const T *c = img + (x + i) * d0 + (y + jj);
@@ -192,7 +196,9 @@ __global__ __launch_bounds__(256) void orb_harris(unsigned char *__restrict__ fl
     }
 }
 
-template <typename T>
+// PATCH is ORB's 31; each patch row's PATCH values are loaded at once, then
+// added in the reference's order (row by row, column by column).
+template <typename T, int PATCH>
 __global__ __launch_bounds__(256) void orb_angle(float *__restrict__ ox, float *__restrict__ oy,
                                                  float *__restrict__ oscore, float *__restrict__ oori,
                                                  const T *__restrict__ img, const unsigned *__restrict__ sel1,
@@ -200,8 +206,9 @@ __global__ __launch_bounds__(256) void orb_angle(float *__restrict__ ox, float *
                                                  const unsigned *__restrict__ sel3,
                                                  const unsigned *__restrict__ order,
                                                  const float *__restrict__ keys, long long n, int d0, int d1,
-                                                 int edge, int nx, int patch) {
+                                                 int edge, int nx) {
 #pragma clang fp contract(off)
+    constexpr int patch = PATCH;
     for (long long f = blockIdx.x * (long long)blockDim.x + threadIdx.x; f < n;
          f += (long long)gridDim.x * blockDim.x) {
         const unsigned k = sel1[sel2[sel3[order[f]]]];
@@ -213,17 +220,22 @@ __global__ __launch_bounds__(256) void orb_angle(float *__restrict__ ox, float *
         // column past the image, so it is skipped here too (features reach
         // this kernel only from the Harris step, which keeps them 22 pixels
         // inside, so neither test ever fires for them).
-        const int r = patch / 2;
+        constexpr int r = patch / 2;
         if (x < r || y < r || x + r >= (long long)d1 || y + r >= (long long)d0) {
             oori[f] = 0.f;
             continue;
         }
         T m01 = T(0), m10 = T(0);
-        for (int q = 0; q < patch * patch; ++q) {
-            const int i = q / patch - r, jj = q % patch - r;
-            const T p = img[(x + i) * d0 + (y + jj)];
-            m01 = m01 + (T)jj * p;
-            m10 = m10 + (T)i * p;
+        for (int ii = 0; ii < patch; ++ii) {
+            const T *col = img + (x + ii - r) * d0 + (y - r);
+            T v[patch];
+#pragma unroll
+            for (int jj = 0; jj < patch; ++jj) v[jj] = col[jj];
+#pragma unroll
+            for (int jj = 0; jj < patch; ++jj) {
+                m01 = m01 + (T)(jj - r) * v[jj];
+                m10 = m10 + (T)(ii - r) * v[jj];
+            }
         }
         oori[f] = (float)atan2((double)m01, (double)m10);
     }
@@ -254,6 +266,7 @@ __global__ __launch_bounds__(256) void orb_describe(unsigned *__restrict__ desc,
         const float s = (float)sin((double)ori), c = (float)cos((double)ori);
         const float pscl = (float)patch / (float)patch;
         unsigned v = 0u;
+#pragma unroll
         for (int j = 0; j < 32; ++j) {
             const signed char *pp = kOrbPairs[w * 32 + j];
             const float ax = (float)pp[0] * pscl, ay = (float)pp[1] * pscl;
@@ -328,15 +341,15 @@ extern "C" int afhip_k_orb_harris(int dtype, unsigned char *flags, float *resp, 
                                   long long d1, int edge, int patch_r, int block, float k_thr, void *stream) {
     const long long nx = d1 - 2LL * edge;
     if (n < 1 || nx < 1 || d0 * d1 >= (1LL << 31)) return K_ERR_SIZE;
-    if (block < 1 || patch_r < block / 2 + 1) return K_ERR_ARG;
+    if (block != 7 || patch_r < block / 2 + 1) return K_ERR_ARG;
     hipStream_t s = (hipStream_t)stream;
     const unsigned g = orb_grid(n);
     if (dtype == DT_F32)
-        orb_harris<float><<<g, 256, 0, s>>>(flags, resp, (const float *)img, sel1, sel2, n, (int)d0, (int)d1, edge,
-                                            (int)nx, patch_r, block, k_thr);
+        orb_harris<float, 7><<<g, 256, 0, s>>>(flags, resp, (const float *)img, sel1, sel2, n, (int)d0, (int)d1,
+                                               edge, (int)nx, patch_r, k_thr);
     else if (dtype == DT_F64)
-        orb_harris<double><<<g, 256, 0, s>>>(flags, resp, (const double *)img, sel1, sel2, n, (int)d0, (int)d1,
-                                             edge, (int)nx, patch_r, block, k_thr);
+        orb_harris<double, 7><<<g, 256, 0, s>>>(flags, resp, (const double *)img, sel1, sel2, n, (int)d0, (int)d1,
+                                                edge, (int)nx, patch_r, k_thr);
     else
         return K_ERR_TYPE;
     return launch_status();
@@ -348,15 +361,15 @@ extern "C" int afhip_k_orb_angle(int dtype, float *x, float *y, float *score, fl
                                  int edge, int patch, void *stream) {
     const long long nx = d1 - 2LL * edge;
     if (n < 1 || nx < 1 || d0 * d1 >= (1LL << 31)) return K_ERR_SIZE;
-    if (patch < 1 || patch / 2 > edge) return K_ERR_ARG;
+    if (patch != 31 || patch / 2 > edge) return K_ERR_ARG;
     hipStream_t s = (hipStream_t)stream;
     const unsigned g = orb_grid(n);
     if (dtype == DT_F32)
-        orb_angle<float><<<g, 256, 0, s>>>(x, y, score, ori, (const float *)img, sel1, sel2, sel3, order, keys, n,
-                                           (int)d0, (int)d1, edge, (int)nx, patch);
+        orb_angle<float, 31><<<g, 256, 0, s>>>(x, y, score, ori, (const float *)img, sel1, sel2, sel3, order, keys,
+                                               n, (int)d0, (int)d1, edge, (int)nx);
     else if (dtype == DT_F64)
-        orb_angle<double><<<g, 256, 0, s>>>(x, y, score, ori, (const double *)img, sel1, sel2, sel3, order, keys, n,
-                                            (int)d0, (int)d1, edge, (int)nx, patch);
+        orb_angle<double, 31><<<g, 256, 0, s>>>(x, y, score, ori, (const double *)img, sel1, sel2, sel3, order,
+                                                keys, n, (int)d0, (int)d1, edge, (int)nx);
     else
         return K_ERR_TYPE;
     return launch_status();

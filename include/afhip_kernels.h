@@ -441,9 +441,9 @@ AFHIP_KAPI int afhip_k_canny_grad(float *gx, float *gy, float *mag,
 AFHIP_KAPI int afhip_k_canny_nms(float *out, const float *mag, const float *gx,
                                  const float *gy, long long d0, long long d1,
                                  long long ns, void *stream);
-/* mm[0] = min, mm[1] = max of n values. */
+/* mm[0] = min, mm[1] = max of n values; part: 2048 floats of scratch. */
 AFHIP_KAPI int afhip_k_canny_minmax(float *mm, const float *v, long long n,
-                                    void *stream);
+                                    float *part, void *stream);
 /* Per slice: the histogram of nbins over [0, maxval] into hist (zeroed by
  * the caller) and Otsu's threshold (a bin index) into thr. */
 AFHIP_KAPI int afhip_k_canny_otsu(float *thr, unsigned *hist, const float *v,
