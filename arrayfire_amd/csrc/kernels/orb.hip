@@ -122,9 +122,16 @@ __global__ __launch_bounds__(256) void orb_zero(float *__restrict__ v, long long
         v[e] = 0.f;
 }
 
+// The corner count in device memory, capped at n (the launch's bound).
+__device__ inline long long capped(const unsigned *count, long long n) {
+    return count ? min(n, (long long)*count) : n;
+}
+
 __global__ __launch_bounds__(256) void orb_fast_scatter(float *__restrict__ V, const unsigned *__restrict__ sel,
-                                                        const float *__restrict__ score, long long n, int d0,
-                                                        int edge, int nx) {
+                                                        const float *__restrict__ score, long long n,
+                                                        const unsigned *__restrict__ count, int d0, int edge,
+                                                        int nx) {
+    n = capped(count, n);
     for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < n;
          j += (long long)gridDim.x * blockDim.x) {
         const unsigned k = sel[j];
@@ -135,10 +142,16 @@ __global__ __launch_bounds__(256) void orb_fast_scatter(float *__restrict__ V, c
 
 __global__ __launch_bounds__(256) void orb_fast_nonmax(unsigned char *__restrict__ flags,
                                                        const float *__restrict__ V,
-                                                       const unsigned *__restrict__ sel, long long n, int d0,
-                                                       int d1, int edge, int nx) {
+                                                       const unsigned *__restrict__ sel, long long n,
+                                                       const unsigned *__restrict__ count, int d0, int d1, int edge,
+                                                       int nx) {
+    const long long ne = capped(count, n);
     for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < n;
          j += (long long)gridDim.x * blockDim.x) {
+        if (j >= ne) {
+            flags[j] = 0;
+            continue;
+        }
         const unsigned k = sel[j];
         const long long y = edge + (long long)(k / (unsigned)nx), x = edge + (long long)(k % (unsigned)nx);
         const float *c = V + x * d0 + y;
@@ -162,12 +175,19 @@ __global__ __launch_bounds__(256) void orb_fast_nonmax(unsigned char *__restrict
 template <typename T, int BLOCK>
 __global__ __launch_bounds__(256) void orb_harris(unsigned char *__restrict__ flags, float *__restrict__ resp,
                                                   const T *__restrict__ img, const unsigned *__restrict__ sel1,
-                                                  const unsigned *__restrict__ sel2, long long n, int d0, int d1,
-                                                  int edge, int nx, int patch_r, float k_thr) {
+                                                  const unsigned *__restrict__ sel2, long long n,
+                                                  const unsigned *__restrict__ count, int d0, int d1, int edge,
+                                                  int nx, int patch_r, float k_thr) {
 #pragma clang fp contract(off)
     constexpr int block = BLOCK;
+    const long long ne = capped(count, n);
     for (long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x; j < n;
          j += (long long)gridDim.x * blockDim.x) {
+        if (j >= ne) {
+            flags[j] = 0;
+            resp[j] = 0.f;
+            continue;
+        }
         const unsigned k = sel1[sel2[j]];
         const long long y = edge + (long long)(k / (unsigned)nx), x = edge + (long long)(k % (unsigned)nx);
         if (x < patch_r || y < patch_r || x >= (long long)d1 - patch_r || y >= (long long)d0 - patch_r) {
@@ -326,30 +346,32 @@ extern "C" int afhip_k_orb_fast(int dtype, unsigned char *flags, float *score, c
 }
 
 extern "C" int afhip_k_orb_fast_nonmax(unsigned char *flags, float *V, const unsigned *sel, const float *score,
-                                       long long n, long long d0, long long d1, int edge, void *stream) {
+                                       long long n, const unsigned *count, long long d0, long long d1, int edge,
+                                       void *stream) {
     const long long nx = d1 - 2LL * edge;
     if (n < 1 || nx < 1 || edge < 1 || d0 * d1 >= (1LL << 31)) return K_ERR_SIZE;
     hipStream_t s = (hipStream_t)stream;
     orb_zero<<<orb_grid(d0 * d1), 256, 0, s>>>(V, d0 * d1);
-    orb_fast_scatter<<<orb_grid(n), 256, 0, s>>>(V, sel, score, n, (int)d0, edge, (int)nx);
-    orb_fast_nonmax<<<orb_grid(n), 256, 0, s>>>(flags, V, sel, n, (int)d0, (int)d1, edge, (int)nx);
+    orb_fast_scatter<<<orb_grid(n), 256, 0, s>>>(V, sel, score, n, count, (int)d0, edge, (int)nx);
+    orb_fast_nonmax<<<orb_grid(n), 256, 0, s>>>(flags, V, sel, n, count, (int)d0, (int)d1, edge, (int)nx);
     return launch_status();
 }
 
 extern "C" int afhip_k_orb_harris(int dtype, unsigned char *flags, float *resp, const void *img,
-                                  const unsigned *sel1, const unsigned *sel2, long long n, long long d0,
-                                  long long d1, int edge, int patch_r, int block, float k_thr, void *stream) {
+                                  const unsigned *sel1, const unsigned *sel2, long long n, const unsigned *count,
+                                  long long d0, long long d1, int edge, int patch_r, int block, float k_thr,
+                                  void *stream) {
     const long long nx = d1 - 2LL * edge;
     if (n < 1 || nx < 1 || d0 * d1 >= (1LL << 31)) return K_ERR_SIZE;
     if (block != 7 || patch_r < block / 2 + 1) return K_ERR_ARG;
     hipStream_t s = (hipStream_t)stream;
     const unsigned g = orb_grid(n);
     if (dtype == DT_F32)
-        orb_harris<float, 7><<<g, 256, 0, s>>>(flags, resp, (const float *)img, sel1, sel2, n, (int)d0, (int)d1,
-                                               edge, (int)nx, patch_r, k_thr);
+        orb_harris<float, 7><<<g, 256, 0, s>>>(flags, resp, (const float *)img, sel1, sel2, n, count, (int)d0,
+                                               (int)d1, edge, (int)nx, patch_r, k_thr);
     else if (dtype == DT_F64)
-        orb_harris<double, 7><<<g, 256, 0, s>>>(flags, resp, (const double *)img, sel1, sel2, n, (int)d0, (int)d1,
-                                                edge, (int)nx, patch_r, k_thr);
+        orb_harris<double, 7><<<g, 256, 0, s>>>(flags, resp, (const double *)img, sel1, sel2, n, count, (int)d0,
+                                                (int)d1, edge, (int)nx, patch_r, k_thr);
     else
         return K_ERR_TYPE;
     return launch_status();

@@ -2,10 +2,10 @@
 // image pyramid, FAST corners, Harris ranking, centroid orientation and
 // rotated-BRIEF descriptors, all computed on the GPU (kernels/orb.hip); each
 // level's descriptor source is the level smoothed by the separable
-// convolution path with a 9-tap sigma-2 Gaussian. The host reads three
-// counts per level (corners found, corners kept by suppression, corners
-// inside the Harris window) to size the next launches, where the reference
-// synchronises its queue.
+// convolution path with a 9-tap sigma-2 Gaussian. The corner counts after
+// detection and after suppression stay on the device (the kernels after
+// them run over the level's cap and stop at the count); the host reads one
+// count per level, the sort's size, and synchronises once at the end.
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -45,24 +45,34 @@ std::vector<T> gauss_taps() {
     return h;
 }
 
+// Buffers of one af_orb call, released together after its last sync.
+typedef std::vector<std::shared_ptr<Buffer>> Keep;
+
+std::shared_ptr<Buffer> kept(Keep &keep, size_t bytes) {
+    keep.push_back(scratch(bytes));
+    return keep.back();
+}
+
+// Stable compaction of the set flags: indices into sel; the count stays in
+// device memory (the returned buffer).
+std::shared_ptr<Buffer> select(Keep &keep, const std::shared_ptr<Buffer> &flags, long long n,
+                               std::shared_ptr<Buffer> &sel) {
+    const long long tb = afhip_k_select_flagged_workspace(n);
+    if (tb < 0) AFHIP_THROW(AF_ERR_INTERNAL, "orb: select workspace");
+    sel = kept(keep, sizeof(unsigned) * (size_t)n);
+    auto num = kept(keep, sizeof(unsigned));
+    auto temp = kept(keep, (size_t)(tb > 0 ? tb : 1));
+    check_k(afhip_k_select_flagged((const unsigned char *)flags->ptr, n, (unsigned *)sel->ptr, (unsigned *)num->ptr,
+                                   temp->ptr, tb, stream()),
+            "orb");
+    return num;
+}
+
 unsigned read_count(const std::shared_ptr<Buffer> &num) {
     unsigned v = 0;
     check_hip(hipMemcpyAsync(&v, num->ptr, sizeof(unsigned), hipMemcpyDeviceToHost, stream()), "orb");
     sync_stream();
     return v;
-}
-
-// Stable compaction of the set flags: indices into sel, count returned.
-unsigned select(const std::shared_ptr<Buffer> &flags, long long n, std::shared_ptr<Buffer> &sel) {
-    const long long tb = afhip_k_select_flagged_workspace(n);
-    if (tb < 0) AFHIP_THROW(AF_ERR_INTERNAL, "orb: select workspace");
-    sel = scratch(sizeof(unsigned) * (size_t)n);
-    auto num = scratch(sizeof(unsigned));
-    auto temp = scratch((size_t)(tb > 0 ? tb : 1));
-    check_k(afhip_k_select_flagged((const unsigned char *)flags->ptr, n, (unsigned *)sel->ptr, (unsigned *)num->ptr,
-                                   temp->ptr, tb, stream()),
-            "orb");
-    return read_count(num);  // also keeps temp alive until the select has run
 }
 
 struct Level {
@@ -71,8 +81,8 @@ struct Level {
 };
 
 template <typename T>
-Level orb_level(const ArrayImpl &img_arr, const void *img, dim_t d0, dim_t d1, float fast_thr, unsigned lvl_best,
-                float lvl_scl, const std::vector<T> *gauss) {
+Level orb_level(Keep &keep, const ArrayImpl &img_arr, const void *img, dim_t d0, dim_t d1, float fast_thr,
+                unsigned lvl_best, float lvl_scl, const std::vector<T> *gauss) {
     const int dt = img_arr.type;
     hipStream_t s = stream();
     Level L;
@@ -82,52 +92,49 @@ Level orb_level(const ArrayImpl &img_arr, const void *img, dim_t d0, dim_t d1, f
     const int edge = (int)std::ceil(size * std::sqrt(2.f) / 2.f);
     const dim_t ny = d0 - 2 * (dim_t)edge, nx = d1 - 2 * (dim_t)edge;
     if (ny < 1 || nx < 1) return L;
-    // fast(): at most ceil(elements * 0.15) corners are stored
+    // fast(): at most ceil(elements * 0.15) corners are stored; the counts
+    // of the first two compactions stay on the device, the kernels after
+    // them run over this cap and stop at the count
     const unsigned max_fast = (unsigned)std::ceil(static_cast<float>(d0 * d1) * kFeatRatio);
-    auto flags1 = scratch((size_t)(ny * nx));
-    auto score1 = scratch(sizeof(float) * (size_t)(ny * nx));
+    const long long ncap = std::min<long long>(max_fast, ny * nx);
+    auto flags1 = kept(keep, (size_t)(ny * nx));
+    auto score1 = kept(keep, sizeof(float) * (size_t)(ny * nx));
     check_k(afhip_k_orb_fast(dt, (unsigned char *)flags1->ptr, (float *)score1->ptr, img, d0, d1, edge, fast_thr,
                              kArc, s),
             "orb");
     std::shared_ptr<Buffer> sel1, sel2, sel3;
-    unsigned n1 = select(flags1, ny * nx, sel1);
-    if (n1 > max_fast) n1 = max_fast;
-    if (n1 == 0) return L;
-    auto V = scratch(sizeof(float) * (size_t)(d0 * d1));
-    auto flags2 = scratch(n1);
+    auto num1 = select(keep, flags1, ny * nx, sel1);
+    auto V = kept(keep, sizeof(float) * (size_t)(d0 * d1));
+    auto flags2 = kept(keep, (size_t)ncap);
     check_k(afhip_k_orb_fast_nonmax((unsigned char *)flags2->ptr, (float *)V->ptr, (const unsigned *)sel1->ptr,
-                                    (const float *)score1->ptr, n1, d0, d1, edge, s),
+                                    (const float *)score1->ptr, ncap, (const unsigned *)num1->ptr, d0, d1, edge, s),
             "orb");
-    const unsigned n2 = select(flags2, n1, sel2);
-    if (n2 == 0) return L;
+    auto num2 = select(keep, flags2, ncap, sel2);
     // harris_response<T, false> over a 7x7 block, patch radius as FAST's edge
-    auto flags3 = scratch(n2);
-    auto resp3 = scratch(sizeof(float) * n2);
+    auto flags3 = kept(keep, (size_t)ncap);
+    auto resp3 = kept(keep, sizeof(float) * (size_t)ncap);
     check_k(afhip_k_orb_harris(dt, (unsigned char *)flags3->ptr, (float *)resp3->ptr, img,
-                               (const unsigned *)sel1->ptr, (const unsigned *)sel2->ptr, n2, d0, d1, edge, edge,
-                               kHarrisBlock, kHarrisK, s),
+                               (const unsigned *)sel1->ptr, (const unsigned *)sel2->ptr, ncap,
+                               (const unsigned *)num2->ptr, d0, d1, edge, edge, kHarrisBlock, kHarrisK, s),
             "orb");
-    const unsigned n3 = select(flags3, n2, sel3);
+    const unsigned n3 = read_count(select(keep, flags3, ncap, sel3));
     if (n3 == 0) return L;
     // sort_index(..., descending): stable descending radix sort of the
     // responses; the strongest lvl_best are kept
     const long long tb = afhip_k_sort_desc_workspace(n3);
     if (tb < 0) AFHIP_THROW(AF_ERR_INTERNAL, "orb: sort workspace");
-    auto keys = scratch(sizeof(float) * n3), vals = scratch(sizeof(unsigned) * n3);
-    auto ksort = scratch(sizeof(float) * n3), vsort = scratch(sizeof(unsigned) * n3);
-    auto ttemp = scratch((size_t)(tb > 0 ? tb : 1));
+    auto keys = kept(keep, sizeof(float) * n3), vals = kept(keep, sizeof(unsigned) * n3);
+    auto ksort = kept(keep, sizeof(float) * n3), vsort = kept(keep, sizeof(unsigned) * n3);
+    auto ttemp = kept(keep, (size_t)(tb > 0 ? tb : 1));
     check_k(afhip_k_harris_sort((const unsigned *)sel3->ptr, (const float *)resp3->ptr, n3, (float *)keys->ptr,
                                 (unsigned *)vals->ptr, (float *)ksort->ptr, (unsigned *)vsort->ptr, ttemp->ptr, tb,
                                 s),
             "orb");
     const unsigned usable = n3 < lvl_best ? n3 : lvl_best;
-    if (usable == 0) {
-        sync_stream();
-        return L;
-    }
-    auto lx = scratch(sizeof(float) * usable), ly = scratch(sizeof(float) * usable);
-    L.score = scratch(sizeof(float) * usable);
-    L.ori = scratch(sizeof(float) * usable);
+    if (usable == 0) return L;
+    auto lx = kept(keep, sizeof(float) * usable), ly = kept(keep, sizeof(float) * usable);
+    L.score = kept(keep, sizeof(float) * usable);
+    L.ori = kept(keep, sizeof(float) * usable);
     check_k(afhip_k_orb_angle(dt, (float *)lx->ptr, (float *)ly->ptr, (float *)L.score->ptr, (float *)L.ori->ptr,
                               img, (const unsigned *)sel1->ptr, (const unsigned *)sel2->ptr,
                               (const unsigned *)sel3->ptr, (const unsigned *)vsort->ptr,
@@ -136,12 +143,9 @@ Level orb_level(const ArrayImpl &img_arr, const void *img, dim_t d0, dim_t d1, f
     // the descriptor source: the level smoothed by convolve2(img, g, g) in
     // the reference's two-pass order, or the level itself
     const void *src = img;
-    std::shared_ptr<Buffer> filt, tmp, taps;
     if (gauss) {
         const size_t bytes = sizeof(T) * (size_t)(d0 * d1);
-        filt = scratch(bytes);
-        tmp = scratch(bytes);
-        taps = scratch(sizeof(T) * kGaussLen);
+        auto filt = kept(keep, bytes), tmp = kept(keep, bytes), taps = kept(keep, sizeof(T) * kGaussLen);
         check_hip(hipMemcpyAsync(taps->ptr, gauss->data(), sizeof(T) * kGaussLen, hipMemcpyHostToDevice, s), "orb");
         const long long dd[4] = {d0, d1, 1, 1};
         check_k(afhip_k_convolve2_sep(dt, filt->ptr, dd, img, dd, taps->ptr, kGaussLen, taps->ptr, kGaussLen, 0,
@@ -149,16 +153,15 @@ Level orb_level(const ArrayImpl &img_arr, const void *img, dim_t d0, dim_t d1, f
                 "orb");
         src = filt->ptr;
     }
-    L.x = scratch(sizeof(float) * usable);
-    L.y = scratch(sizeof(float) * usable);
-    L.size = scratch(sizeof(float) * usable);
-    L.desc = scratch(sizeof(unsigned) * 8 * (size_t)usable);
+    L.x = kept(keep, sizeof(float) * usable);
+    L.y = kept(keep, sizeof(float) * usable);
+    L.size = kept(keep, sizeof(float) * usable);
+    L.desc = kept(keep, sizeof(unsigned) * 8 * (size_t)usable);
     const int r = (int)std::ceil(kPatch * std::sqrt(2.f) / 2.f);
     check_k(afhip_k_orb_describe(dt, (unsigned *)L.desc->ptr, (float *)L.x->ptr, (float *)L.y->ptr,
                                  (float *)L.size->ptr, (const float *)lx->ptr, (const float *)ly->ptr,
                                  (const float *)L.ori->ptr, src, usable, d0, d1, (int)kPatch, r, lvl_scl, s),
             "orb");
-    sync_stream();  // the level's scratch goes back to the pool on return
     L.n = usable;
     return L;
 }
@@ -178,6 +181,7 @@ void orb(Features &feat, af_array &desc_out, const ArrayImpl &in, float fast_thr
         scl_sum += 1.f / std::pow(scl_fctr, static_cast<float>(i));
     }
     std::vector<Level> lv;
+    Keep keep;  // every level's buffers, released after the final sync
     if (max_levels > 0) {
         // features per level: ceil(max_feat / scl_sum / scale), the rest to
         // the last level (unsigned arithmetic, as the reference's)
@@ -201,7 +205,7 @@ void orb(Features &feat, af_array &desc_out, const ArrayImpl &in, float fast_thr
             if (i > 0) {
                 d0 = (dim_t)std::round(static_cast<float>(id0) / ls);
                 d1 = (dim_t)std::round(static_cast<float>(id1) / ls);
-                cur = scratch(sizeof(T) * (size_t)(d0 * d1));
+                cur = kept(keep, sizeof(T) * (size_t)(d0 * d1));
                 check_k(afhip_k_orb_resize(in.type, cur->ptr, prev_ptr, pd0, pd1, d0, d1, stream()), "orb");
                 img = cur->ptr;
                 prev = cur;
@@ -209,7 +213,7 @@ void orb(Features &feat, af_array &desc_out, const ArrayImpl &in, float fast_thr
                 pd0 = d0;
                 pd1 = d1;
             }
-            Level L = orb_level<T>(in, img, d0, d1, fast_thr, lvl_best[i], ls, blur_img ? &g : nullptr);
+            Level L = orb_level<T>(keep, in, img, d0, d1, fast_thr, lvl_best[i], ls, blur_img ? &g : nullptr);
             if (L.n > 0) lv.push_back(std::move(L));
         }
     }
@@ -237,7 +241,7 @@ void orb(Features &feat, af_array &desc_out, const ArrayImpl &in, float fast_thr
                   "orb");
         off += L.n;
     }
-    sync_stream();  // the levels' buffers go back to the pool on return
+    sync_stream();  // the levels' buffers (keep) go back to the pool on return
     feat.n = total;
     feat.x = hx.release();
     feat.y = hy.release();

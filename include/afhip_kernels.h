@@ -479,20 +479,23 @@ AFHIP_KAPI int afhip_k_orb_resize(int dtype, void *out, const void *in,
 AFHIP_KAPI int afhip_k_orb_fast(int dtype, unsigned char *flags, float *score,
                                 const void *img, long long d0, long long d1,
                                 int edge, float thr, int arc, void *stream);
-/* Suppression of the n corners sel[] (scan ranks) on the score map V
- * (d0 x d1 floats, cleared here): flags[j] keeps corner j. */
+/* Suppression of the first min(n, *count) corners sel[] (scan ranks; count
+ * in device memory, or null for n) on the score map V (d0 x d1 floats,
+ * cleared here): flags[j] keeps corner j; flags past the count are 0. */
 AFHIP_KAPI int afhip_k_orb_fast_nonmax(unsigned char *flags, float *V,
                                        const unsigned *sel, const float *score,
-                                       long long n, long long d0, long long d1,
-                                       int edge, void *stream);
-/* Harris response of the n corners sel1[sel2[j]] over a block x block
- * window; flags[j] = 0 for corners within patch_r of a border. */
+                                       long long n, const unsigned *count,
+                                       long long d0, long long d1, int edge,
+                                       void *stream);
+/* Harris response of the first min(n, *count) corners sel1[sel2[j]] (count
+ * in device memory, or null for n) over a 7 x 7 block window; flags[j] = 0
+ * for corners within patch_r of a border and past the count. */
 AFHIP_KAPI int afhip_k_orb_harris(int dtype, unsigned char *flags, float *resp,
                                   const void *img, const unsigned *sel1,
                                   const unsigned *sel2, long long n,
-                                  long long d0, long long d1, int edge,
-                                  int patch_r, int block, float k_thr,
-                                  void *stream);
+                                  const unsigned *count, long long d0,
+                                  long long d1, int edge, int patch_r,
+                                  int block, float k_thr, void *stream);
 /* The n kept features sel1[sel2[sel3[order[f]]]] with scores keys[f]:
  * positions and intensity-centroid angle over a patch x patch window. */
 AFHIP_KAPI int afhip_k_orb_angle(int dtype, float *x, float *y, float *score,
