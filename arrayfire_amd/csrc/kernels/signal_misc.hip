@@ -119,12 +119,34 @@ __global__ __launch_bounds__(64) void iir_kernel(T *__restrict__ y, const T *__r
         T ac[NZ];
 #pragma unroll
         for (int q = 0; q < NZ; ++q) ac[q] = q < na ? aa[q] : T(0);
-        for (long long i = 0; i < g.yd[0]; ++i) {
-            const T v = O::div(O::add(cc[i], z[0]), ac[0]);
-            yy[i] = v;
+        // the inputs arrive in chunks of CHK loaded a chunk ahead into the
+        // other of two register buffers, so the recurrence never waits on a
+        // single load; the arithmetic and its order are unchanged
+        constexpr int CHK = sizeof(T) <= 4 ? 32 : (sizeof(T) <= 8 ? 16 : 8);
+        const long long L = g.yd[0];
+        T A[CHK], B[CHK];
+        auto load = [&](long long i0, T (&dst)[CHK]) __attribute__((always_inline)) {
 #pragma unroll
-            for (int q = 1; q < NZ; ++q)
-                if (q < na) z[q - 1] = O::sub(z[q], O::mul(ac[q], v));
+            for (int q = 0; q < CHK; ++q) dst[q] = i0 + q < L ? cc[i0 + q] : T(0);
+        };
+        auto run = [&](long long i0, const T (&src)[CHK]) __attribute__((always_inline)) {
+#pragma unroll
+            for (int q = 0; q < CHK; ++q) {
+                if (i0 + q >= L) break;
+                const T v = O::div(O::add(src[q], z[0]), ac[0]);
+                yy[i0 + q] = v;
+#pragma unroll
+                for (int r = 1; r < NZ; ++r)
+                    if (r < na) z[r - 1] = O::sub(z[r], O::mul(ac[r], v));
+            }
+        };
+        load(0, A);
+        for (long long i0 = 0; i0 < L; i0 += 2 * CHK) {
+            if (i0 + CHK < L) load(i0 + CHK, B);
+            run(i0, A);
+            if (i0 + CHK >= L) break;
+            if (i0 + 2 * CHK < L) load(i0 + 2 * CHK, A);
+            run(i0 + CHK, B);
         }
     } else {
         T *z = zw + col * g.na;

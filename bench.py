@@ -713,6 +713,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy (cast_kernel<float, float>) 5 times, "
                          "to scale rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--callers-only", action="store_true",
+                    help="only run the frequency-domain route and the callers (secondary_callers) and print "
+                         "that line, for kernel traces of them")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -733,6 +736,9 @@ def main():
             assert rc == 0
         af.sync()
         print(json.dumps({"copy_calibration_bytes_each_way": 4 * n, "launches": 5}))
+        return
+    if args.callers_only:
+        print(json.dumps({"secondary_callers": callers_secondary(af, cpu=not args.no_cpu)}))
         return
     ev = HipEvents()
     rng = np.random.default_rng(SEED)
