@@ -208,7 +208,7 @@ def test_conv2_consistency(af, rng, sd, fd):
     native = af.convolve2(x, w, af.AF_CONV_DEFAULT, af.AF_CONV_AUTO).to_numpy()
     nn = af.convolve2NN(x, w, (1, 1), (fd[0] // 2, fd[1] // 2), (1, 1)).to_numpy()
     assert native.shape == nn.shape
-    assert np.max(np.abs(native - nn)) <= 2e-5 * max(1.0, np.max(np.abs(native)) / 10)
+    assert np.max(np.abs(native - nn)) <= 2e-5
 
 
 def test_zero_padding_issue2817(af):
