@@ -182,6 +182,9 @@ void orb(Features &feat, af_array &desc_out, const ArrayImpl &in, float fast_thr
     }
     std::vector<Level> lv;
     Keep keep;  // every level's buffers, released after the final sync
+    // the blur taps: host memory that asynchronous copies read, alive until
+    // the final sync
+    const std::vector<T> g = blur_img ? gauss_taps<T>() : std::vector<T>();
     if (max_levels > 0) {
         // features per level: ceil(max_feat / scl_sum / scale), the rest to
         // the last level (unsigned arithmetic, as the reference's)
@@ -193,7 +196,6 @@ void orb(Features &feat, af_array &desc_out, const ArrayImpl &in, float fast_thr
             feat_sum += lvl_best[i];
         }
         lvl_best[max_levels - 1] = max_feat - feat_sum;
-        const std::vector<T> g = blur_img ? gauss_taps<T>() : std::vector<T>();
         std::shared_ptr<Buffer> prev;  // level i-1 (level 0 is the input)
         const void *prev_ptr = in.data();
         dim_t pd0 = id0, pd1 = id1;

@@ -13,7 +13,7 @@ within an ulp of a half-integer may round the other way.
 
 Cases: f32 and f64 images, with and without the Gaussian blur, scale
 factors 1.2 / 1.5 / 2 and level counts that stop at the patch size, a
-single-level image, an image too small for a level (no features), and the
+768x1024 image over five levels, a single-level image, an image too small for a level (no features), and the
 reference's argument checks."""
 import numpy as np
 import pytest
@@ -41,6 +41,8 @@ CASES = [
     ((160, 200), np.float64, 15.0, 300, 2.0, 3, True),
     ((160, 200), np.float64, 15.0, 150, 1.5, 8, False),
     ((60, 70), np.float32, 10.0, 100, 1.5, 4, True),
+    # a camera-sized image: five levels, the full feature budget
+    ((768, 1024), np.float32, 20.0, 500, 1.5, 5, True),
 ]
 
 
