@@ -26,8 +26,9 @@
 //    angle (centroid_angle, src/backend/cpu/kernel/orb.hpp:164-193), the two
 //    moments accumulated in T in patch order by one lane each;
 //  * orb_describe: the 256-bit rotated-BRIEF descriptor
-//    (extract_orb/get_pixel, src/backend/cpu/kernel/orb.hpp:195-264), one
-//    lane per 32-bit word, and the positions scaled to the level's scale.
+//    (extract_orb/get_pixel, src/backend/cpu/kernel/orb.hpp:195-264), a lane
+//    per point pair, the words assembled by wave ballots, and the positions
+//    scaled to the level's scale.
 //
 // Every product and sum is rounded separately (no contraction), as the
 // reference's CPU build evaluates them. Angles are atan2 of the moments and
@@ -261,6 +262,9 @@ __global__ __launch_bounds__(256) void orb_angle(float *__restrict__ ox, float *
     }
 }
 
+// One workgroup of 256 lanes per feature, one lane per point pair (bit b of
+// the descriptor compares pair b); each wave's 64 comparisons become two
+// 32-bit words by a ballot.
 template <typename T>
 __global__ __launch_bounds__(256) void orb_describe(unsigned *__restrict__ desc, float *__restrict__ ox,
                                                     float *__restrict__ oy, float *__restrict__ osize,
@@ -268,42 +272,42 @@ __global__ __launch_bounds__(256) void orb_describe(unsigned *__restrict__ desc,
                                                     const float *__restrict__ fori, const T *__restrict__ img,
                                                     long long n, int d0, int d1, int patch, int r, float scl) {
 #pragma clang fp contract(off)
-    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n * 8;
-         t += (long long)gridDim.x * blockDim.x) {
-        const long long f = t >> 3;
-        const int w = (int)(t & 7);
-        const long long x = (long long)roundf(fx[f]), y = (long long)roundf(fy[f]);
-        if (x < r || y < r || x >= (long long)d1 - r || y >= (long long)d0 - r) {
-            desc[t] = 0u;
-            if (w == 0) {
-                ox[f] = fx[f];
-                oy[f] = fy[f];
-                osize[f] = 0.f;
-            }
-            continue;
+    const long long f = blockIdx.x;
+    if (f >= n) return;  // uniform per workgroup
+    const int b = threadIdx.x, lane = b & 63, wv = b >> 6;
+    const long long x = (long long)roundf(fx[f]), y = (long long)roundf(fy[f]);
+    if (x < r || y < r || x >= (long long)d1 - r || y >= (long long)d0 - r) {
+        if (lane == 0) {
+            desc[f * 8 + 2 * wv] = 0u;
+            desc[f * 8 + 2 * wv + 1] = 0u;
         }
-        const float ori = fori[f];
-        const float s = (float)sin((double)ori), c = (float)cos((double)ori);
-        const float pscl = (float)patch / (float)patch;
-        unsigned v = 0u;
-#pragma unroll
-        for (int j = 0; j < 32; ++j) {
-            const signed char *pp = kOrbPairs[w * 32 + j];
-            const float ax = (float)pp[0] * pscl, ay = (float)pp[1] * pscl;
-            const float bx = (float)pp[2] * pscl, by = (float)pp[3] * pscl;
-            const long long x1 = x + (long long)roundf(ax * c - ay * s);
-            const long long y1 = y + (long long)roundf(ax * s + ay * c);
-            const long long x2 = x + (long long)roundf(bx * c - by * s);
-            const long long y2 = y + (long long)roundf(bx * s + by * c);
-            const T p1 = img[x1 * d0 + y1], p2 = img[x2 * d0 + y2];
-            v |= (unsigned)(p1 < p2) << j;
+        if (b == 0) {
+            ox[f] = fx[f];
+            oy[f] = fy[f];
+            osize[f] = 0.f;
         }
-        desc[t] = v;
-        if (w == 0) {
-            ox[f] = roundf((float)x * scl);
-            oy[f] = roundf((float)y * scl);
-            osize[f] = (float)patch * scl;
-        }
+        return;
+    }
+    const float ori = fori[f];
+    const float sn = (float)sin((double)ori), cs = (float)cos((double)ori);
+    const float pscl = (float)patch / (float)patch;
+    const signed char *pp = kOrbPairs[b];
+    const float ax = (float)pp[0] * pscl, ay = (float)pp[1] * pscl;
+    const float bx = (float)pp[2] * pscl, by = (float)pp[3] * pscl;
+    const long long x1 = x + (long long)roundf(ax * cs - ay * sn);
+    const long long y1 = y + (long long)roundf(ax * sn + ay * cs);
+    const long long x2 = x + (long long)roundf(bx * cs - by * sn);
+    const long long y2 = y + (long long)roundf(bx * sn + by * cs);
+    const T p1 = img[x1 * d0 + y1], p2 = img[x2 * d0 + y2];
+    const unsigned long long m = __ballot(p1 < p2);
+    if (lane == 0) {
+        desc[f * 8 + 2 * wv] = (unsigned)(m & 0xffffffffull);
+        desc[f * 8 + 2 * wv + 1] = (unsigned)(m >> 32);
+    }
+    if (b == 0) {
+        ox[f] = roundf((float)x * scl);
+        oy[f] = roundf((float)y * scl);
+        osize[f] = (float)patch * scl;
     }
 }
 
@@ -404,7 +408,8 @@ extern "C" int afhip_k_orb_describe(int dtype, unsigned *desc, float *ox, float 
     // every sample stays inside the image: |offset| <= 13*sqrt(2) < 19 < r
     if (patch != 31 || r < 19) return K_ERR_ARG;
     hipStream_t s = (hipStream_t)stream;
-    const unsigned g = orb_grid(n * 8);
+    if (n >= (1LL << 31)) return K_ERR_SIZE;
+    const unsigned g = (unsigned)n;  // a workgroup per feature, a lane per point pair
     if (dtype == DT_F32)
         orb_describe<float><<<g, 256, 0, s>>>(desc, ox, oy, osize, x, y, ori, (const float *)img, n, (int)d0,
                                               (int)d1, patch, r, scl);
