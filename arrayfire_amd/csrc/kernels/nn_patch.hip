@@ -1422,6 +1422,14 @@ static bool plan_wres(WresParams &w, int &nch, int &nq, int &nf, const PatchPara
             }
         }
         if (best < 0) return false;
+        // AFHIP_NN_WRES_RB=<rows> in the environment forces the band height
+        // (within the LDS budget)
+        if (const char *rb_env = std::getenv("AFHIP_NN_WRES_RB")) {
+            const int rb = std::atoi(rb_env);
+            if (rb >= 1 && rb <= p.OH &&
+                (size_t)nch * ((rb - 1) * p.s1 + 2 * p.d1 + 1) * w.PW * Q_PB <= kPatchLdsLimit)
+                w.RB = rb;
+        }
         w.PR = (w.RB - 1) * p.s1 + 2 * p.d1 + 1;
         w.RBa = (w.RB + 1) / 2;
         w.PRa = (w.RBa - 1) * p.s1 + 2 * p.d1 + 1;
